@@ -1,0 +1,243 @@
+"""Benchmark: batched H psi on one MI355X.
+
+Workload (the headline configuration): periodic 96^3 grid, h = 0.18 A,
+256 bands of smoothed Gaussian noise, local potential of 32 Gaussian wells,
+32 atoms with l = 0..2 Gaussian projectors (9 per atom) and random
+symmetric dH.  One step = H psi for all 256 bands:
+projections P = <p|psi>, c = P dH, then T psi + v psi + sum p c in one
+stencil launch.  Metric: band * gridpoints / s (higher is better).
+
+Prints one JSON line.  Also reported: the FP64 path, the multigrid
+preconditioner on the same grid, the stencil kernel's roofline (HBM bytes
+per launch / measured launch time) and the CPU oracle on a bounded sample
+of bands (all host cores, FP64).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--N', type=int, default=96)
+    p.add_argument('--bands', type=int, default=256)
+    p.add_argument('--atoms', type=int, default=32)
+    p.add_argument('--no-cpu', action='store_true',
+                   help='skip the CPU oracle baseline')
+    p.add_argument('--cpu-seconds', type=float, default=12.0,
+                   help='target length of the CPU baseline sample')
+    return p.parse_args()
+
+
+def time_events(fn, steps, stream_sync):
+    """Run fn() `steps` times; return (wall seconds, per-call ms list of the
+    events fn records)."""
+    import torch
+    stream_sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    stream_sync()
+    return time.perf_counter() - t0
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU; --gpus %d is not '
+              'supported (bands shard across GPUs with no communication for '
+              'H psi, see the design notes)' % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gpawdft_amd.hamiltonian import Hamiltonian
+    from gpawdft_amd.preconditioner import Preconditioner
+    from gpawdft_amd.synthetic import System, smooth_bands_device
+
+    assert torch.cuda.is_available(), 'bench.py needs a HIP device'
+    dev = torch.device('cuda')
+    N, nb = args.N, args.bands
+    sysm = System(N, 0.18, args.atoms, seed=2024)
+    gd = sysm.gd
+    G = int(np.prod(gd.n_c))
+    ham = Hamiltonian(gd, [sysm.vt_G], sysm.dH_asp)
+    sync = torch.cuda.synchronize
+
+    results = {}
+    for dt, name in ((torch.float32, 'f32'), (torch.float64, 'f64')):
+        kin, pt = sysm.operators()
+        psi = smooth_bands_device(nb, gd.n_c, 7, dtype=dt)
+        out = torch.empty_like(psi)
+        P = torch.zeros((nb, pt.Mmax), dtype=dt, device=dev)
+        c = torch.empty_like(P)
+        ham.apply_fused(psi, out, kin, pt, P_xM=P, c_xM=c)   # tables + warm
+        for _ in range(args.warmup):
+            ham.apply_fused(psi, out, kin, pt, P_xM=P, c_xM=c)
+        # whole steps, wall clock between two synchronisations
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ham.apply_fused(psi, out, kin, pt, P_xM=P, c_xM=c)
+        sync()
+        wall = time.perf_counter() - t0
+        # per-kernel timing with events on the launch stream
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        t_int = t_stencil = 0.0
+        from gpawdft_amd._lib import check, lib
+        from gpawdft_amd.device import stream_ptr, DeviceView
+        vt = ham._vt(0, dt, dev)
+        for _ in range(args.steps):
+            ev[0].record()
+            P.zero_()
+            pt.integrate_xM(psi, P)
+            ham.projector_coefs(pt, P, 0, c)
+            ev[1].record()
+            check(lib().gpd_hamiltonian_apply(
+                kin.handle, pt.handle, DeviceView(psi).code, nb,
+                psi.data_ptr(), vt.data_ptr(), c.data_ptr(), out.data_ptr(),
+                stream_ptr()), 'hamiltonian_apply')
+            ev[2].record()
+            ev[2].synchronize()
+            t_int += ev[0].elapsed_time(ev[1])
+            t_stencil += ev[1].elapsed_time(ev[2])
+        ms_step = wall / args.steps * 1e3
+        ms_stencil = t_stencil / args.steps
+        ms_proj = t_int / args.steps
+        esize = 4 if dt == torch.float32 else 8
+        alg_bytes = 2 * esize * nb * G
+        results[name] = dict(
+            ms_per_step=ms_step,
+            value=nb * G / (ms_step * 1e-3),
+            ms_stencil=ms_stencil,
+            ms_projections=ms_proj,
+            stencil_gbs=alg_bytes / (ms_stencil * 1e-3) / 1e9,
+            alg_bytes=alg_bytes)
+        # multigrid preconditioner on the same grid, blocks of 16 bands
+        blk = 16
+        pc = Preconditioner(gd, kin, block=blk)
+        R = psi[:blk]
+        d = torch.empty_like(R)
+        for _ in range(2):
+            pc(R, None, out=d)
+        sync()
+        t0 = time.perf_counter()
+        nrep = max(1, args.steps // 2)
+        for _ in range(nrep):
+            for b0 in range(0, nb, blk):
+                pc(psi[b0:b0 + blk], None, out=d)
+        sync()
+        tpc = (time.perf_counter() - t0) / nrep
+        results[name]['precond_ms'] = tpc * 1e3
+        results[name]['precond_value'] = nb * G / tpc
+        del psi, out, P, c, pc, R, d
+        torch.cuda.empty_cache()
+
+    # CPU baseline: the FP64 oracle (all host cores) on a bounded sample
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sysm, args.cpu_seconds)
+
+    f32 = results['f32']
+    traffic = None
+    tf = os.path.join(ROOT, 'profiles', 'hpsi_stencil_96x256_f32_traffic.json')
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get('hbm_bytes_per_launch')
+    line = {
+        'metric': 'band*gridpts/s for H*psi at 96^3x256',
+        'value': f32['value'],
+        'unit': 'band*gridpts/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': f32['ms_per_step'],
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32',
+        'data': 'synthetic (seeded Gaussian-well potential, Gaussian '
+                'projectors, smoothed N(0,1) bands)',
+        'config': {'workload': 'H*psi, 96^3 periodic grid, h=0.18 A, %d '
+                               'bands, %d atoms x 9 projectors, nn=3 stencil'
+                               % (nb, args.atoms),
+                   'grid': [N, N, N], 'bands': nb, 'atoms': args.atoms},
+        'roofline': {'bound': 'hbm',
+                     'kernel': 'fd_axis_kernel<float,3,EPI_POT,LFC> '
+                               '(stencil + v psi + projector add)',
+                     'achieved': f32['stencil_gbs'],
+                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': f32['stencil_gbs'] / HBM_PEAK_GBS,
+                     'traffic': traffic,
+                     'alg_bytes_per_launch': f32['alg_bytes'],
+                     'ms_per_launch': f32['ms_stencil']},
+        'cpu_baseline': cpu,
+        'f32_detail': {k: f32[k] for k in ('ms_stencil', 'ms_projections',
+                                           'precond_ms', 'precond_value')},
+        'f64': {k: results['f64'][k] for k in
+                ('value', 'ms_per_step', 'ms_stencil', 'ms_projections',
+                 'stencil_gbs', 'precond_ms', 'precond_value')},
+    }
+    if cpu:
+        line['speedup_vs_cpu_f32'] = f32['value'] / cpu['value']
+        line['speedup_vs_cpu_f64'] = results['f64']['value'] / cpu['value']
+    print(json.dumps(line))
+
+
+def cpu_baseline(sysm, target_s):
+    """FP64 oracle H psi (stencil, potential, projectors) on all host
+    cores, over as many bands as fit in ~target_s seconds."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    import oracle as O
+    from gpawdft_amd.hamiltonian import unpack
+    from gpawdft_amd.synthetic import smooth_bands_numpy
+    gd = sysm.gd
+    kin_c, kin_o = None, None
+    from gpawdft_amd.fd_operators import guc_laplace_stencil
+    coefs, offs, _ = guc_laplace_stencil(gd, -0.5, sysm.nn)
+    from gpawdft_amd.lfc import LFC
+    pt = LFC(gd, [sysm.splines] * sysm.natoms)
+    pt.set_positions(sysm.spos_ac)
+    atoms = [(pt.M_a[a], unpack(sysm.dH_asp[a][0])) for a in pt.atom_indices]
+    threads = O.num_threads()
+
+    def run(psi):
+        t = O.fd(coefs, offs, gd.pbc_c, psi)
+        t = O.add_local_potential(psi, sysm.vt_G, t)
+        P = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, psi,
+                            pt.Mmax)
+        c = O.projector_coefs(P, atoms)
+        return O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c, t)
+
+    nb = max(threads, 1)
+    psi = smooth_bands_numpy(nb, gd.n_c, 11)
+    t0 = time.perf_counter()
+    run(psi)
+    t1 = time.perf_counter() - t0
+    reps = max(1, int(target_s / max(t1, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run(psi)
+    t = (time.perf_counter() - t0) / reps
+    G = int(np.prod(gd.n_c))
+    return {'value': nb * G / t, 'unit': 'band*gridpts/s', 'cores': threads,
+            'kind': 'port',
+            'sample': '%d bands x %d repeats of FP64 H*psi (oracle: stencil '
+                      '+ potential + projector integrate/dH/add) on the same '
+                      '96^3 system' % (nb, reps)}
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,136 @@
+"""ctypes binding of libgpawdft_amd.so, the gfx950 C ABI declared in
+include/gpawdft_amd.h.
+
+The library is the only compute path: if it is missing, importing any
+operator raises instead of falling back to CPU code.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+GPD_OK = 0
+GPD_ERR_ARG = -1
+GPD_ERR_HIP = -2
+GPD_ERR_UNSUPPORTED = -3
+GPD_ERR_NOMEM = -4
+GPD_F64 = 0
+GPD_F32 = 1
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        'libgpawdft_amd.so')
+
+_c_long_p = ctypes.POINTER(ctypes.c_long)
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_vpp = ctypes.POINTER(ctypes.c_void_p)
+_i = ctypes.c_int
+_l = ctypes.c_long
+_d = ctypes.c_double
+
+_SIGNATURES = {
+    'gpd_last_error': (ctypes.c_char_p, []),
+    'gpd_version': (ctypes.c_char_p, []),
+    'gpd_operator_create': (_i, [_c_double_p, _c_long_p, _i, _c_long_p, _i,
+                                 _c_long_p, _i, _i, _vpp]),
+    'gpd_operator_apply': (_i, [_vp, _i, _l, _vp, _vp, _vp, _vp]),
+    'gpd_operator_get_diagonal_element': (_i, [_vp, _c_double_p]),
+    'gpd_operator_destroy': (_i, [_vp]),
+    'gpd_transformer_create': (_i, [_c_long_p, _c_long_p, _i, _c_long_p,
+                                    _c_long_p, _c_long_p, _c_long_p, _i, _i,
+                                    _vpp]),
+    'gpd_transformer_apply': (_i, [_vp, _i, _l, _vp, _vp, _vp, _vp]),
+    'gpd_transformer_destroy': (_i, [_vp]),
+    'gpd_spline_create': (_i, [_i, _d, _c_double_p, _i, _vpp]),
+    'gpd_spline_value': (_d, [_vp, _d]),
+    'gpd_spline_get_cutoff': (_d, [_vp]),
+    'gpd_spline_get_angular_momentum_number': (_i, [_vp]),
+    'gpd_spline_destroy': (_i, [_vp]),
+    'gpd_spline_to_grid': (_i, [_vp, _c_long_p, _c_long_p, _c_double_p,
+                                _c_double_p, _c_long_p, _c_long_p,
+                                _c_double_p, _c_int_p, _c_long_p, _c_long_p]),
+    'gpd_lfc_create': (_i, [_i, _c_double_p, _c_int_p, _c_int_p, _c_int_p, _i,
+                            _c_int_p, _c_int_p, _d, _l, _i, _vpp]),
+    'gpd_lfc_integrate': (_i, [_vp, _i, _l, _vp, _vp, _vp]),
+    'gpd_lfc_add': (_i, [_vp, _i, _l, _vp, _vp, _vp]),
+    'gpd_lfc_destroy': (_i, [_vp]),
+    'gpd_hamiltonian_apply': (_i, [_vp, _vp, _i, _l, _vp, _vp, _vp, _vp,
+                                   _vp]),
+    'gpd_add_local_potential': (_i, [_i, _l, _l, _vp, _vp, _vp, _vp]),
+    'gpd_projector_coefs': (_i, [_i, _l, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp]),
+    'gpd_precond_create': (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _l,
+                                _vpp]),
+    'gpd_precond_apply': (_i, [_vp, _i, _l, _vp, _vp, _vp]),
+    'gpd_precond_destroy': (_i, [_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                'gpawdft_amd: %s is missing; build it with '
+                '`make -C gpawdft_amd/csrc` (hipcc, gfx950)' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def release(fn_name, handle):
+    """Destroy a library object; safe during interpreter shutdown."""
+    L = _lib
+    if L is None or handle is None or not handle.value:
+        return
+    try:
+        getattr(L, fn_name)(handle)
+    except Exception:
+        pass
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+class GPDError(RuntimeError):
+    pass
+
+
+def check(rc, what=''):
+    if rc == GPD_OK:
+        return
+    msg = lib().gpd_last_error().decode(errors='replace')
+    text = '%s failed (status %d): %s' % (what, rc, msg)
+    if rc == GPD_ERR_ARG:
+        raise ValueError(text)
+    if rc == GPD_ERR_UNSUPPORTED:
+        raise NotImplementedError(text)
+    if rc == GPD_ERR_NOMEM:
+        raise MemoryError(text)
+    raise GPDError(text)
+
+
+def as_long3(x):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.int64).reshape(-1))
+    return a
+
+
+def ptr_long(a):
+    return a.ctypes.data_as(_c_long_p)
+
+
+def ptr_double(a):
+    return a.ctypes.data_as(_c_double_p)
+
+
+def ptr_int(a):
+    return a.ctypes.data_as(_c_int_p)

@@ -1,0 +1,83 @@
+// Shared definitions for the gfx950 kernels of the grid Hamiltonian path.
+//
+// Every source in this library is compiled with -ffp-contract=off: an
+// FP64 expression a*b + c is always two roundings, as in the reference's
+// C code, so the FP64 kernels reproduce its bits.  FP32 kernels ask for
+// fused multiply-adds explicitly through madd<float>.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gpawdft_amd.h"
+
+namespace gpd {
+
+// madd(s, a, b) = s + a*b.  double: two roundings (bit-compatible with the
+// reference).  float: one fused rounding.
+template <typename T> __device__ __forceinline__ T madd(T s, T a, T b);
+template <> __device__ __forceinline__ double madd<double>(double s, double a, double b)
+{
+    return s + a * b;
+}
+template <> __device__ __forceinline__ float madd<float>(float s, float a, float b)
+{
+    return __builtin_fmaf(a, b, s);
+}
+
+// Wrap index i (which may be negative or >= n) into [0, n) for a periodic
+// axis; return -1 for a non-periodic axis when i is outside the grid.
+__device__ __forceinline__ int wrap_index(int i, int n, int periodic)
+{
+    if (i >= 0 && i < n) return i;
+    if (!periodic) return -1;
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+inline int last_error_to_status(hipError_t e)
+{
+    return e == hipSuccess ? GPD_OK : GPD_ERR_HIP;
+}
+
+// Number of XCDs on MI355X; workgroups are dispatched round-robin over them.
+constexpr int kNumXcd = 8;
+
+// Map a hardware block index to a work index so that consecutive work
+// indices land on the same XCD (and share its L2).  Returns -1 for the
+// padding blocks of the last round.
+__device__ __forceinline__ long xcd_work_index(long bid, long nwork)
+{
+    long per = (nwork + kNumXcd - 1) / kNumXcd;
+    long x = bid % kNumXcd;
+    long k = bid / kNumXcd;
+    long w = x * per + k;
+    return (k < per && w < nwork) ? w : -1;
+}
+
+inline long xcd_grid_size(long nwork)
+{
+    long per = (nwork + kNumXcd - 1) / kNumXcd;
+    return per * kNumXcd;
+}
+
+void set_last_error(const char* msg);
+
+}  // namespace gpd
+
+#define GPD_CHECK_HIP(expr)                                   \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) {                               \
+            gpd::set_last_error(hipGetErrorString(e_));       \
+            return GPD_ERR_HIP;                               \
+        }                                                     \
+    } while (0)
+
+#define GPD_REQUIRE(cond, code, msg)                          \
+    do {                                                      \
+        if (!(cond)) {                                        \
+            gpd::set_last_error(msg);                         \
+            return (code);                                    \
+        }                                                     \
+    } while (0)

@@ -1,0 +1,539 @@
+// Finite-difference operators on periodic / zero-bounded grids (gfx950).
+//
+// Behaviour follows _gpaw.Operator (c/operators.c:316-405) with the halo of
+// c/bc.c and the stencil sum of c/bmgs/fd.c:32-46: out[p] = sum_c coef_c *
+// in[p + off_c], accumulated from 0.0 in coefficient order, periodic axes
+// wrapped and non-periodic axes zero outside the grid.  No padded copy is
+// made: the halo is resolved while loading.
+//
+// Two kernels:
+//  * fd_axis_kernel: stencils whose points lie on the three axes in the
+//    order the GUCLaplace constructor emits (centre, then per axis
+//    +1..+R, -1..-R).  A workgroup owns a 16 x 32 tile of the (n1, n2)
+//    plane and streams along axis 0; axis-0 neighbours live in a register
+//    window, in-plane neighbours in an LDS tile with an R-wide halo ring.
+//    Fused epilogues serve the Hamiltonian (local potential and projector
+//    terms) and the multigrid preconditioner.
+//  * fd_generic_kernel: any stencil (off-axis points of non-orthorhombic
+//    cells), one thread per point, all loads through the cache.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "fd.hpp"
+
+namespace gpd {
+
+// ------------------------------------------------------------------ //
+// generic kernel
+
+template <typename T>
+__global__ void fd_generic_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                  int n0, int n1, int n2, int pbc0, int pbc1,
+                                  int pbc2, int ncoefs,
+                                  const int* __restrict__ off,
+                                  const T* __restrict__ coefs)
+{
+    long nG = (long)n0 * n1 * n2;
+    long band = blockIdx.y;
+    const T* a = in + band * nG;
+    T* b = out + band * nG;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nG;
+         p += (long)gridDim.x * blockDim.x) {
+        int i2 = (int)(p % n2);
+        long t = p / n2;
+        int i1 = (int)(t % n1);
+        int i0 = (int)(t / n1);
+        T s = T(0);
+        for (int c = 0; c < ncoefs; c++) {
+            int j0 = wrap_index(i0 + off[3 * c], n0, pbc0);
+            int j1 = wrap_index(i1 + off[3 * c + 1], n1, pbc1);
+            int j2 = wrap_index(i2 + off[3 * c + 2], n2, pbc2);
+            T v = T(0);
+            if (j0 >= 0 && j1 >= 0 && j2 >= 0)
+                v = a[((long)j0 * n1 + j1) * n2 + j2];
+            s = madd<T>(s, v, coefs[c]);
+        }
+        b[p] = s;
+    }
+}
+
+// ------------------------------------------------------------------ //
+// axis kernel
+
+constexpr int kTX = 32;        // tile width along axis 2 (contiguous)
+constexpr int kTY = 16;        // tile height along axis 1
+constexpr int kRows = 2;       // rows per thread: blockDim = 32 x 8
+constexpr int kThreads = 256;
+
+template <typename T, int R> struct AxisCoefs {
+    T c0;
+    T c[3][2 * R];   // per axis block: +1..+R then -1..-R
+    int order[3];    // axis of each block
+};
+
+template <typename T, int R, int EPI, bool LFC>
+__global__ __launch_bounds__(kThreads) void fd_axis_kernel(
+    const T* __restrict__ in, T* __restrict__ out, int n0, int n1, int n2,
+    int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<T, R> cf,
+    FdEpilogue<T> ep)
+{
+    constexpr int LW = kTX + 2 * R;
+    constexpr int LH = kTY + 2 * R;
+    constexpr int NRING = 2 * R * LW + 2 * R * kTY;
+    constexpr int RING_PER_THREAD = (NRING + kThreads - 1) / kThreads;
+    __shared__ T lds[LH][LW];
+
+    const int tiles1 = (n1 + kTY - 1) / kTY;
+    const int tiles2 = (n2 + kTX - 1) / kTX;
+    const int zchunks = (n0 + tz - 1) / tz;
+    const long nwork = (long)nband * zchunks * tiles1 * tiles2;
+    const long w = xcd_work_index(blockIdx.x, nwork);
+    if (w < 0) return;
+    int t2 = (int)(w % tiles2);
+    long r = w / tiles2;
+    int t1 = (int)(r % tiles1);
+    r /= tiles1;
+    int zc = (int)(r % zchunks);
+    long band = r / zchunks;
+
+    const int tx = threadIdx.x % kTX;
+    const int ty = threadIdx.x / kTX;          // 0..7
+    const int x0 = t2 * kTX, y0 = t1 * kTY;
+    const int z0 = zc * tz, z1 = min(n0, z0 + tz);
+    const long plane = (long)n1 * n2;
+    const long nG = plane * n0;
+    const T* a = in + band * nG;
+    T* b = out + band * nG;
+
+    // own points: rows ty and ty + 8 of the tile
+    int gx = x0 + tx;
+    bool valid[kRows];
+    long own_off[kRows];      // in-plane offset, or -1 when zero
+    for (int k = 0; k < kRows; k++) {
+        int gy = y0 + ty + 8 * k;
+        valid[k] = gx < n2 && gy < n1;
+        int wy = wrap_index(gy, n1, pbc1);
+        int wx = wrap_index(gx, n2, pbc2);
+        own_off[k] = (wy >= 0 && wx >= 0) ? (long)wy * n2 + wx : -1;
+    }
+    // ring slots of this thread
+    int ring_i[RING_PER_THREAD], ring_j[RING_PER_THREAD];
+    long ring_off[RING_PER_THREAD];
+    for (int k = 0; k < RING_PER_THREAD; k++) {
+        int s = threadIdx.x + k * kThreads;
+        int i = -1, j = -1;
+        if (s < NRING) {
+            if (s < R * LW) {                       // top rows
+                i = s / LW; j = s % LW;
+            } else if (s < 2 * R * LW) {            // bottom rows
+                s -= R * LW;
+                i = kTY + R + s / LW; j = s % LW;
+            } else {                                // side columns
+                s -= 2 * R * LW;
+                i = R + s / (2 * R);
+                int c = s % (2 * R);
+                j = c < R ? c : kTX + c;
+            }
+        }
+        ring_i[k] = i;
+        ring_j[k] = j;
+        long o = -1;
+        if (i >= 0) {
+            int wy = wrap_index(y0 - R + i, n1, pbc1);
+            int wx = wrap_index(x0 - R + j, n2, pbc2);
+            if (wy >= 0 && wx >= 0) o = (long)wy * n2 + wx;
+        }
+        ring_off[k] = o;
+    }
+
+    auto load_plane = [&](int z, long off) -> T {
+        int wz = wrap_index(z, n0, pbc0);
+        if (wz < 0 || off < 0) return T(0);
+        return a[(long)wz * plane + off];
+    };
+
+    // register window: win[k][j] = value at plane z - R + j
+    T win[kRows][2 * R + 1];
+    for (int k = 0; k < kRows; k++)
+        for (int j = 0; j < 2 * R + 1; j++)
+            win[k][j] = (valid[k] && j < 2 * R) ? load_plane(z0 - R + j, own_off[k]) : T(0);
+
+    for (int z = z0; z < z1; z++) {
+        // leading edge
+        for (int k = 0; k < kRows; k++)
+            if (valid[k]) win[k][2 * R] = load_plane(z + R, own_off[k]);
+        // fill LDS with plane z
+        for (int k = 0; k < kRows; k++) {
+            T v = valid[k] ? win[k][R] : load_plane(z, own_off[k]);
+            lds[R + ty + 8 * k][R + tx] = v;
+        }
+        for (int k = 0; k < RING_PER_THREAD; k++)
+            if (ring_i[k] >= 0) lds[ring_i[k]][ring_j[k]] = load_plane(z, ring_off[k]);
+        __syncthreads();
+
+        for (int k = 0; k < kRows; k++) {
+            if (!valid[k]) continue;
+            const int li = R + ty + 8 * k, lj = R + tx;
+            const T x = win[k][R];
+            T s = T(0);
+            s = madd<T>(s, x, cf.c0);
+#pragma unroll
+            for (int blk = 0; blk < 3; blk++) {
+                const int ax = cf.order[blk];
+                if (ax == 0) {
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, win[k][R + j], cf.c[blk][j - 1]);
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, win[k][R - j], cf.c[blk][R + j - 1]);
+                } else if (ax == 1) {
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li + j][lj], cf.c[blk][j - 1]);
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li - j][lj], cf.c[blk][R + j - 1]);
+                } else {
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li][lj + j], cf.c[blk][j - 1]);
+#pragma unroll
+                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li][lj - j], cf.c[blk][R + j - 1]);
+                }
+            }
+            const long p = (long)z * plane + own_off[k];
+            T o;
+            if (EPI == EPI_PLAIN) {
+                o = s;
+            } else if (EPI == EPI_POT) {
+                o = madd<T>(s, x, ep.aux[p]);
+                if (LFC) {
+                    const int e0 = ep.lfc_ptr[p], e1 = ep.lfc_ptr[p + 1];
+                    const T* cb = ep.lfc_c + band * ep.Mmax;
+                    for (int e = e0; e < e1; e++) {
+                        const int2 en = ep.lfc_ent[e];
+                        const int nm = en.y & 15;
+                        const T* Ar = ep.lfc_A + en.x;
+                        const T* cm = cb + (en.y >> 4);
+                        for (int m = 0; m < nm; m++) o = madd<T>(o, Ar[m], cm[m]);
+                    }
+                }
+            } else if (EPI == EPI_SUB) {
+                o = s - ep.aux[band * nG + p];
+            } else {
+                T q = s - ep.aux[band * nG + p];
+                o = madd<T>(x, ep.alpha, q);
+                if (ep.neg) o = -o;
+            }
+            b[p] = o;
+        }
+        __syncthreads();
+        for (int k = 0; k < kRows; k++)
+#pragma unroll
+            for (int j = 0; j < 2 * R; j++) win[k][j] = win[k][j + 1];
+    }
+}
+
+// Epilogue for stencils without the axis form: s was computed by the
+// generic kernel into `s_buf`; same per-point arithmetic as the axis kernel.
+template <typename T, int EPI>
+__global__ void fd_epilogue_kernel(const T* __restrict__ s_buf, const T* __restrict__ in,
+                                   T* __restrict__ out, long nG, long nb, FdEpilogue<T> ep)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * nG) return;
+    const long band = idx / nG, p = idx % nG;
+    const T s = s_buf[idx], x = in[idx];
+    T o;
+    if (EPI == EPI_POT) {
+        o = madd<T>(s, x, ep.aux[p]);
+        if (ep.lfc_ptr) {
+            const int e0 = ep.lfc_ptr[p], e1 = ep.lfc_ptr[p + 1];
+            const T* cb = ep.lfc_c + band * ep.Mmax;
+            for (int e = e0; e < e1; e++) {
+                const int2 en = ep.lfc_ent[e];
+                const int nm = en.y & 15;
+                const T* Ar = ep.lfc_A + en.x;
+                const T* cm = cb + (en.y >> 4);
+                for (int m = 0; m < nm; m++) o = madd<T>(o, Ar[m], cm[m]);
+            }
+        }
+    } else if (EPI == EPI_SUB) {
+        o = s - ep.aux[idx];
+    } else {
+        T q = s - ep.aux[idx];
+        o = madd<T>(x, ep.alpha, q);
+        if (ep.neg) o = -o;
+    }
+    out[idx] = o;
+}
+
+// ------------------------------------------------------------------ //
+// host side
+
+template <typename T, int R>
+static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
+{
+    AxisCoefs<T, R> cf;
+    cf.c0 = (T)op->canon_c0;
+    for (int blk = 0; blk < 3; blk++) {
+        cf.order[blk] = op->canon_order[blk];
+        for (int j = 0; j < 2 * R; j++) cf.c[blk][j] = (T)op->canon_c[blk][j];
+    }
+    return cf;
+}
+
+static int pick_tz(int n0, long work_per_plane_chunk, int R)
+{
+    // Enough workgroups to fill 256 CUs several times over, but chunks at
+    // least 4R planes deep so the warm-up of the register window stays cheap.
+    long target = 4096;
+    int nz = (int)std::max<long>(1, std::min<long>(n0 / std::max(4 * R, 8), target / std::max<long>(1, work_per_plane_chunk)));
+    return (n0 + nz - 1) / nz;
+}
+
+template <typename T, int R, int EPI, bool LFC>
+static int launch_axis_R(const gpd_operator* op, long nb, const T* in, T* out,
+                         const FdEpilogue<T>& ep, hipStream_t st)
+{
+    const int n0 = op->n[0], n1 = op->n[1], n2 = op->n[2];
+    const int tiles1 = (n1 + kTY - 1) / kTY, tiles2 = (n2 + kTX - 1) / kTX;
+    const int tz = pick_tz(n0, nb * tiles1 * tiles2, R);
+    const int zchunks = (n0 + tz - 1) / tz;
+    const long nwork = nb * zchunks * (long)tiles1 * tiles2;
+    AxisCoefs<T, R> cf = make_axis_coefs<T, R>(op);
+    hipLaunchKernelGGL((fd_axis_kernel<T, R, EPI, LFC>), dim3(xcd_grid_size(nwork)),
+                       dim3(kThreads), 0, st, in, out, n0, n1, n2, op->pbc[0],
+                       op->pbc[1], op->pbc[2], (int)nb, tz, cf, ep);
+    return last_error_to_status(hipGetLastError());
+}
+
+template <typename T, int EPI, bool LFC>
+static int launch_axis(const gpd_operator* op, long nb, const T* in, T* out,
+                       const FdEpilogue<T>& ep, hipStream_t st)
+{
+    switch (op->range) {
+    case 1: return launch_axis_R<T, 1, EPI, LFC>(op, nb, in, out, ep, st);
+    case 2: return launch_axis_R<T, 2, EPI, LFC>(op, nb, in, out, ep, st);
+    case 3: return launch_axis_R<T, 3, EPI, LFC>(op, nb, in, out, ep, st);
+    case 4: return launch_axis_R<T, 4, EPI, LFC>(op, nb, in, out, ep, st);
+    case 5: return launch_axis_R<T, 5, EPI, LFC>(op, nb, in, out, ep, st);
+    case 6: return launch_axis_R<T, 6, EPI, LFC>(op, nb, in, out, ep, st);
+    default: return GPD_ERR_UNSUPPORTED;
+    }
+}
+
+template <typename T>
+static int launch_generic(const gpd_operator* op, long nb, const T* in, T* out,
+                          hipStream_t st)
+{
+    const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+    const T* coefs = std::is_same<T, double>::value ? (const T*)op->d_coefs64
+                                                    : (const T*)op->d_coefs32;
+    long blocks = std::min<long>((nG + 255) / 256, 4096);
+    for (long b0 = 0; b0 < nb; b0 += 65535) {
+        long nbb = std::min<long>(65535, nb - b0);
+        hipLaunchKernelGGL(fd_generic_kernel<T>, dim3(blocks, nbb), dim3(256), 0, st,
+                           in + b0 * nG, out + b0 * nG, op->n[0], op->n[1], op->n[2],
+                           op->pbc[0], op->pbc[1], op->pbc[2], op->ncoefs,
+                           op->d_off, coefs);
+        int s = last_error_to_status(hipGetLastError());
+        if (s) return s;
+    }
+    return GPD_OK;
+}
+
+template <typename T>
+int fd_apply_typed(const gpd_operator* op, int epi, long nb, const T* in, T* out,
+                   const FdEpilogue<T>& ep, hipStream_t st)
+{
+    if (!op->canonical) {
+        if (epi == EPI_PLAIN) return launch_generic<T>(op, nb, in, out, st);
+        // generic stencil into a stream-ordered temporary, then the epilogue
+        const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+        T* s_buf = nullptr;
+        hipError_t e = hipMallocAsync((void**)&s_buf, sizeof(T) * nb * nG, st);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            return GPD_ERR_HIP;
+        }
+        int rc = launch_generic<T>(op, nb, in, s_buf, st);
+        if (rc == GPD_OK) {
+            long n = nb * nG;
+            dim3 grid((n + 255) / 256);
+            if (epi == EPI_POT)
+                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_POT>), grid, dim3(256), 0, st, s_buf,
+                                   in, out, nG, nb, ep);
+            else if (epi == EPI_SUB)
+                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB>), grid, dim3(256), 0, st, s_buf,
+                                   in, out, nG, nb, ep);
+            else
+                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB_AXPY>), grid, dim3(256), 0, st,
+                                   s_buf, in, out, nG, nb, ep);
+            rc = last_error_to_status(hipGetLastError());
+        }
+        (void)hipFreeAsync(s_buf, st);
+        return rc;
+    }
+    switch (epi) {
+    case EPI_PLAIN: return launch_axis<T, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case EPI_POT:
+        if (ep.lfc_ptr) return launch_axis<T, EPI_POT, true>(op, nb, in, out, ep, st);
+        return launch_axis<T, EPI_POT, false>(op, nb, in, out, ep, st);
+    case EPI_SUB: return launch_axis<T, EPI_SUB, false>(op, nb, in, out, ep, st);
+    case EPI_SUB_AXPY: return launch_axis<T, EPI_SUB_AXPY, false>(op, nb, in, out, ep, st);
+    default: return GPD_ERR_ARG;
+    }
+}
+
+template int fd_apply_typed<double>(const gpd_operator*, int, long, const double*, double*,
+                                    const FdEpilogue<double>&, hipStream_t);
+template int fd_apply_typed<float>(const gpd_operator*, int, long, const float*, float*,
+                                   const FdEpilogue<float>&, hipStream_t);
+
+// Decode a flat offset built with strides [(n1+2r)(n2+2r), n2+2r, 1].
+static bool decode_offset(long flat, const long M[3], int r, int d[3])
+{
+    long t = flat;
+    for (int c = 2; c >= 1; c--) {
+        long m = ((t % M[c]) + M[c]) % M[c];     // 0 .. M-1
+        long dc = m > r ? m - M[c] : m;          // balanced digit
+        d[c] = (int)dc;
+        t = (t - dc) / M[c];
+    }
+    d[0] = (int)t;
+    for (int c = 0; c < 3; c++)
+        if (d[c] < -r || d[c] > r) return false;
+    return true;
+}
+
+// Recognise centre, then per axis +1..+R, -1..-R (gpaw/fd_operators.py:194-202).
+static void detect_canonical(gpd_operator* op)
+{
+    op->canonical = false;
+    const int R = op->range;
+    if (R < 1 || R > 6 || op->ncoefs != 1 + 6 * R) return;
+    const int* o = op->off.data();
+    if (o[0] || o[1] || o[2]) return;
+    bool seen[3] = {false, false, false};
+    int idx = 1;
+    for (int blk = 0; blk < 3; blk++) {
+        const int* first = o + 3 * idx;
+        int ax = -1;
+        for (int c = 0; c < 3; c++)
+            if (first[c] != 0) ax = c;
+        if (ax < 0 || seen[ax]) return;
+        seen[ax] = true;
+        for (int j = 0; j < 2 * R; j++) {
+            const int* e = o + 3 * (idx + j);
+            int want = j < R ? j + 1 : -(j - R + 1);
+            for (int c = 0; c < 3; c++)
+                if (e[c] != (c == ax ? want : 0)) return;
+            op->canon_c[blk][j] = op->coefs[idx + j];
+        }
+        op->canon_order[blk] = ax;
+        idx += 2 * R;
+    }
+    op->canon_c0 = op->coefs[0];
+    op->canonical = true;
+}
+
+}  // namespace gpd
+
+using namespace gpd;
+
+extern "C" int gpd_operator_create(const double* coefs, const long* offsets, int ncoefs,
+                                   const long size[3], int range, const long neighbors[6],
+                                   int real, int cfd, gpd_operator** out)
+{
+    GPD_REQUIRE(out && coefs && offsets && size && neighbors, GPD_ERR_ARG, "null argument");
+    *out = nullptr;
+    GPD_REQUIRE(ncoefs > 0 && range >= 0, GPD_ERR_ARG, "bad stencil size");
+    GPD_REQUIRE(real == 1, GPD_ERR_UNSUPPORTED, "complex operators are not supported");
+    for (int c = 0; c < 3; c++) GPD_REQUIRE(size[c] > 0, GPD_ERR_ARG, "empty grid");
+    gpd_operator* op = new (std::nothrow) gpd_operator();
+    GPD_REQUIRE(op, GPD_ERR_NOMEM, "out of host memory");
+    op->ncoefs = ncoefs;
+    op->range = range;
+    op->cfd = cfd;
+    long M[3];
+    for (int c = 0; c < 3; c++) {
+        op->n[c] = (int)size[c];
+        M[c] = size[c] + 2 * range;
+        // a single rank is its own neighbour on a periodic axis
+        op->pbc[c] = neighbors[2 * c] >= 0 && neighbors[2 * c + 1] >= 0;
+    }
+    op->coefs.assign(coefs, coefs + ncoefs);
+    op->off.resize(3 * ncoefs);
+    op->diag = 0.0;
+    for (int i = 0; i < ncoefs; i++) {
+        int d[3];
+        if (!decode_offset(offsets[i], M, range, d)) {
+            delete op;
+            GPD_REQUIRE(false, GPD_ERR_ARG, "stencil offset outside its range");
+        }
+        for (int c = 0; c < 3; c++) op->off[3 * i + c] = d[c];
+        if (offsets[i] == 0) op->diag = coefs[i];
+    }
+    for (int c = 0; c < 3; c++) {
+        if (op->pbc[c] && op->n[c] < range) {
+            delete op;
+            GPD_REQUIRE(false, GPD_ERR_ARG, "grid shorter than the stencil range");
+        }
+    }
+    detect_canonical(op);
+    std::vector<float> c32(coefs, coefs + ncoefs);
+    hipError_t e = hipMalloc(&op->d_off, sizeof(int) * 3 * ncoefs);
+    if (e == hipSuccess) e = hipMalloc(&op->d_coefs64, sizeof(double) * ncoefs);
+    if (e == hipSuccess) e = hipMalloc(&op->d_coefs32, sizeof(float) * ncoefs);
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_off, op->off.data(), sizeof(int) * 3 * ncoefs, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_coefs64, coefs, sizeof(double) * ncoefs, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_coefs32, c32.data(), sizeof(float) * ncoefs, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        gpd_operator_destroy(op);
+        set_last_error(hipGetErrorString(e));
+        return GPD_ERR_HIP;
+    }
+    *out = op;
+    return GPD_OK;
+}
+
+extern "C" int gpd_operator_apply(gpd_operator* op, int dtype, long nin, const void* d_in,
+                                  void* d_out, const void* phases, void* stream)
+{
+    GPD_REQUIRE(op, GPD_ERR_ARG, "null operator");
+    GPD_REQUIRE(!phases, GPD_ERR_UNSUPPORTED, "Bloch phases are not supported");
+    GPD_REQUIRE(nin >= 0, GPD_ERR_ARG, "negative batch");
+    if (nin == 0) return GPD_OK;
+    GPD_REQUIRE(d_in && d_out && d_in != d_out, GPD_ERR_ARG, "in and out must be distinct device arrays");
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64) {
+        FdEpilogue<double> ep = {};
+        return fd_apply_typed<double>(op, EPI_PLAIN, nin, (const double*)d_in, (double*)d_out, ep, st);
+    }
+    if (dtype == GPD_F32) {
+        FdEpilogue<float> ep = {};
+        return fd_apply_typed<float>(op, EPI_PLAIN, nin, (const float*)d_in, (float*)d_out, ep, st);
+    }
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_operator_get_diagonal_element(const gpd_operator* op, double* diag)
+{
+    GPD_REQUIRE(op && diag, GPD_ERR_ARG, "null argument");
+    *diag = op->diag;
+    return GPD_OK;
+}
+
+extern "C" int gpd_operator_destroy(gpd_operator* op)
+{
+    if (!op) return GPD_OK;
+    if (op->d_off) (void)hipFree(op->d_off);
+    if (op->d_coefs64) (void)hipFree(op->d_coefs64);
+    if (op->d_coefs32) (void)hipFree(op->d_coefs32);
+    delete op;
+    return GPD_OK;
+}

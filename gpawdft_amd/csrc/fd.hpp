@@ -1,0 +1,55 @@
+// Internal interface of the finite-difference operator kernels.
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+struct gpd_operator {
+    int ncoefs = 0;
+    int range = 0;
+    int cfd = 1;
+    int n[3] = {0, 0, 0};
+    int pbc[3] = {0, 0, 0};
+    std::vector<double> coefs;
+    std::vector<int> off;        // (d0, d1, d2) per coefficient
+    double diag = 0.0;
+    // axis-stencil form: centre, then per axis block +1..+R, -1..-R
+    bool canonical = false;
+    double canon_c0 = 0.0;
+    double canon_c[3][12] = {};
+    int canon_order[3] = {0, 1, 2};
+    // device copies for the generic kernel
+    int* d_off = nullptr;
+    double* d_coefs64 = nullptr;
+    float* d_coefs32 = nullptr;
+};
+
+namespace gpd {
+
+// Fused epilogues of the axis kernel.  s is the stencil sum and x the
+// input value at the output point.
+enum : int {
+    EPI_PLAIN = 0,     // out = s
+    EPI_POT = 1,       // out = s + x * v[p] (+ projector terms)
+    EPI_SUB = 2,       // out = s - aux[p]
+    EPI_SUB_AXPY = 3,  // q = s - aux[p]; out = x + alpha * q (negated if neg)
+};
+
+template <typename T> struct FdEpilogue {
+    const T* aux;          // v (EPI_POT, one grid) or r (EPI_SUB*, batched)
+    T alpha;
+    int neg;
+    // projector terms (EPI_POT): per-point CSR over the active volumes
+    const int* lfc_ptr;    // nG + 1, or nullptr for none
+    const int2* lfc_ent;   // {row offset into A, (M << 4) | nm}
+    const T* lfc_A;
+    const T* lfc_c;        // (nb, Mmax)
+    int Mmax;
+};
+
+template <typename T>
+int fd_apply_typed(const gpd_operator* op, int epi, long nb, const T* in, T* out,
+                   const FdEpilogue<T>& ep, hipStream_t st);
+
+}  // namespace gpd

@@ -1,0 +1,134 @@
+// Fused pseudo-Hamiltonian apply and projector coefficients (gfx950).
+//
+// H psi = T psi + v psi + sum_a p_a (P_a dH_a)  -- the composite of
+// FDWaveFunctions.apply_pseudo_hamiltonian (gpaw/wavefunctions/fd.py:77-83)
+// and Hamiltonian.apply (gpaw/hamiltonian.py:426-460).  The kinetic stencil
+// kernel carries the local potential and the projector terms in its
+// epilogue, so psi is read once and H psi written once; the per-point
+// rounding sequence is the reference's (stencil sum, + psi*v, then the
+// projector terms in volume order).
+#include "common.hpp"
+#include "fd.hpp"
+#include "lfc.hpp"
+
+namespace gpd {
+
+template <typename T>
+__global__ void projector_coefs_kernel(long nb, int Mmax, int natoms, const int* __restrict__ M_a,
+                                       const int* __restrict__ ni_a,
+                                       const int* __restrict__ dH_off, const T* __restrict__ dH,
+                                       const T* __restrict__ P, T* __restrict__ c)
+{
+    // one thread per (band, atom)
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * natoms) return;
+    const int a = (int)(idx % natoms);
+    const long x = idx / natoms;
+    const int ni = ni_a[a];
+    const T* Pa = P + x * Mmax + M_a[a];
+    const T* H = dH + dH_off[a];
+    T* ca = c + x * Mmax + M_a[a];
+    for (int j = 0; j < ni; j++) {
+        T s = T(0);
+        for (int i = 0; i < ni; i++) s = madd<T>(s, Pa[i], H[i * ni + j]);
+        ca[j] = s;
+    }
+}
+
+template <typename T>
+__global__ void local_potential_kernel(long nb, long nG, const T* __restrict__ psi,
+                                       const T* __restrict__ v, T* __restrict__ hpsi)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * nG) return;
+    hpsi[idx] = madd<T>(hpsi[idx], psi[idx], v[idx % nG]);
+}
+
+template <typename T>
+static int ham_apply(gpd_operator* kin, gpd_lfc* lfc, long nb, const T* psi, const T* v,
+                     const T* c, T* out, hipStream_t st)
+{
+    FdEpilogue<T> ep = {};
+    ep.aux = v;
+    if (lfc && c) {
+        ep.lfc_ptr = lfc->ptr;
+        ep.lfc_ent = lfc->ent;
+        ep.lfc_A = lfc_values<T>(lfc);
+        ep.lfc_c = c;
+        ep.Mmax = lfc->Mmax;
+    }
+    return fd_apply_typed<T>(kin, EPI_POT, nb, psi, out, ep, st);
+}
+
+}  // namespace gpd
+
+using namespace gpd;
+
+extern "C" int gpd_hamiltonian_apply(gpd_operator* kin, gpd_lfc* lfc, int dtype, long nb,
+                                     const void* d_psi, const void* d_v, const void* d_c,
+                                     void* d_out, void* stream)
+{
+    GPD_REQUIRE(kin, GPD_ERR_ARG, "null kinetic operator");
+    GPD_REQUIRE(nb >= 0, GPD_ERR_ARG, "negative batch");
+    if (nb == 0) return GPD_OK;
+    GPD_REQUIRE(d_psi && d_v && d_out && d_psi != d_out, GPD_ERR_ARG, "bad arrays");
+    if (lfc) {
+        GPD_REQUIRE(lfc->nG == (long)kin->n[0] * kin->n[1] * kin->n[2], GPD_ERR_ARG,
+                    "projector grid does not match the operator grid");
+        GPD_REQUIRE(d_c, GPD_ERR_ARG, "projector coefficients missing");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64)
+        return ham_apply<double>(kin, lfc, nb, (const double*)d_psi, (const double*)d_v,
+                                 (const double*)d_c, (double*)d_out, st);
+    if (dtype == GPD_F32)
+        return ham_apply<float>(kin, lfc, nb, (const float*)d_psi, (const float*)d_v,
+                                (const float*)d_c, (float*)d_out, st);
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_projector_coefs(int dtype, long nb, int Mmax, int natoms, const int* d_M_a,
+                                   const int* d_ni_a, const int* d_dH_off, const void* d_dH,
+                                   const void* d_P, void* d_c, void* stream)
+{
+    GPD_REQUIRE(nb >= 0 && natoms >= 0 && Mmax >= 0, GPD_ERR_ARG, "bad sizes");
+    if (nb == 0 || natoms == 0) return GPD_OK;
+    GPD_REQUIRE(d_M_a && d_ni_a && d_dH_off && d_dH && d_P && d_c && d_P != d_c, GPD_ERR_ARG,
+                "bad arrays");
+    hipStream_t st = (hipStream_t)stream;
+    long n = nb * natoms;
+    if (dtype == GPD_F64) {
+        hipLaunchKernelGGL(projector_coefs_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           nb, Mmax, natoms, d_M_a, d_ni_a, d_dH_off, (const double*)d_dH,
+                           (const double*)d_P, (double*)d_c);
+        return last_error_to_status(hipGetLastError());
+    }
+    if (dtype == GPD_F32) {
+        hipLaunchKernelGGL(projector_coefs_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           nb, Mmax, natoms, d_M_a, d_ni_a, d_dH_off, (const float*)d_dH,
+                           (const float*)d_P, (float*)d_c);
+        return last_error_to_status(hipGetLastError());
+    }
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_add_local_potential(int dtype, long nb, long nG, const void* d_psi,
+                                       const void* d_v, void* d_hpsi, void* stream)
+{
+    GPD_REQUIRE(nb >= 0 && nG > 0, GPD_ERR_ARG, "bad sizes");
+    if (nb == 0) return GPD_OK;
+    GPD_REQUIRE(d_psi && d_v && d_hpsi, GPD_ERR_ARG, "null array");
+    hipStream_t st = (hipStream_t)stream;
+    long n = nb * nG;
+    if (dtype == GPD_F64) {
+        hipLaunchKernelGGL(local_potential_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           nb, nG, (const double*)d_psi, (const double*)d_v, (double*)d_hpsi);
+        return last_error_to_status(hipGetLastError());
+    }
+    if (dtype == GPD_F32) {
+        hipLaunchKernelGGL(local_potential_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           nb, nG, (const float*)d_psi, (const float*)d_v, (float*)d_hpsi);
+        return last_error_to_status(hipGetLastError());
+    }
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}

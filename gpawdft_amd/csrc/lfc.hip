@@ -1,0 +1,461 @@
+// Localized functions (PAW projectors) on the grid: integrate and add
+// (gfx950).
+//
+// Behaviour follows _gpaw.LFC (c/lfc.c:106-186, 361-425, 724-783) for real
+// functions.  The reference sweeps a G-sorted list of boundary events and
+// keeps the set of "active" volumes in between (c/lfc.h:38-87).  Here that
+// sweep runs once, on the host, when the object is created, and is turned
+// into flat device tables:
+//  * per covered grid point, the list of (A row, M, nm) in the sweep's
+//    active-volume order -- add() walks it point by point, so FP64 sums
+//    round exactly as the reference's a_G[G] += A * c loop;
+//  * per function block M, the (G, A row) sequence in the sweep's order --
+//    the FP64 integrate() walks it sequentially per (band, block), so
+//    c_M += (a_G * dv) * A rounds as the reference;
+//  * the same sequences grouped by identical point lists with the A values
+//    laid out densely -- the FP32 integrate() reads each point once for all
+//    functions of an atom and a block of bands, and reduces in LDS.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "lfc.hpp"
+
+namespace gpd {
+
+template <> const double* lfc_values<double>(const gpd_lfc* l) { return l->A64; }
+template <> const float* lfc_values<float>(const gpd_lfc* l) { return l->A32; }
+
+constexpr int kNF = 16;   // functions per group held in registers
+
+// ------------------------------------------------------------------ //
+// kernels
+
+// FP64-exact integrate: one thread per (band, function block).
+template <typename T>
+__global__ void lfc_integrate_seq_kernel(const T* __restrict__ a, T* __restrict__ c, long nG,
+                                         int Mmax, int nkeys, long nx, T dv,
+                                         const T* __restrict__ A,
+                                         const int* __restrict__ key_M,
+                                         const int* __restrict__ key_nm,
+                                         const long* __restrict__ key_off,
+                                         const int* __restrict__ key_G,
+                                         const int* __restrict__ key_row)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * nkeys) return;
+    const int k = (int)(idx % nkeys);
+    const long band = idx / nkeys;
+    const int M = key_M[k], nm = key_nm[k];
+    const T* ab = a + band * nG;
+    T* cb = c + band * Mmax + M;
+    T acc[kNF];
+#pragma unroll
+    for (int m = 0; m < kNF; m++) acc[m] = m < nm ? cb[m] : T(0);
+    const long j1 = key_off[k + 1];
+    for (long j = key_off[k]; j < j1; j++) {
+        const T av = ab[key_G[j]] * dv;
+        const T* Ar = A + key_row[j];
+#pragma unroll
+        for (int m = 0; m < kNF; m++)
+            if (m < nm) acc[m] = madd<T>(acc[m], av, Ar[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < kNF; m++)
+        if (m < nm) cb[m] = acc[m];
+}
+
+// Grouped integrate: one workgroup per (group, block of NBB bands).
+template <typename T, int NBB>
+__global__ __launch_bounds__(256) void lfc_integrate_grp_kernel(
+    const T* __restrict__ a, T* __restrict__ c, long nG, int Mmax, long nx, T dv,
+    int ngroups, int nfmax, const long* __restrict__ grp_off, const int* __restrict__ grp_nf,
+    const int* __restrict__ grp_fM, const int* __restrict__ grp_G,
+    const float* __restrict__ grp_A)
+{
+    __shared__ float red[4][NBB * kNF];
+    const long nbb = (nx + NBB - 1) / NBB;
+    const long w = xcd_work_index(blockIdx.x, nbb * ngroups);
+    if (w < 0) return;
+    const int g = (int)(w % ngroups);
+    const long b0 = (w / ngroups) * NBB;
+    const int nf = grp_nf[g];
+    const long j0 = grp_off[g], j1 = grp_off[g + 1];
+
+    float acc[NBB][kNF];
+#pragma unroll
+    for (int bb = 0; bb < NBB; bb++)
+#pragma unroll
+        for (int f = 0; f < kNF; f++) acc[bb][f] = 0.f;
+
+    for (long j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        const long G = grp_G[j];
+        const float* Ar = grp_A + j * nfmax;
+        float av[kNF];
+#pragma unroll
+        for (int f = 0; f < kNF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+#pragma unroll
+        for (int bb = 0; bb < NBB; bb++) {
+            const long band = b0 + bb;
+            const float x = band < nx ? (float)a[band * nG + G] : 0.f;
+#pragma unroll
+            for (int f = 0; f < kNF; f++) acc[bb][f] = __builtin_fmaf(x, av[f], acc[bb][f]);
+        }
+    }
+    // reduce over the wave, then over the 4 waves
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+    for (int bb = 0; bb < NBB; bb++)
+#pragma unroll
+        for (int f = 0; f < kNF; f++) {
+            float v = acc[bb][f];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) red[wave][bb * kNF + f] = v;
+        }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NBB * kNF; t += blockDim.x) {
+        const int bb = t / kNF, f = t % kNF;
+        const long band = b0 + bb;
+        if (f < nf && band < nx) {
+            float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+            T* cp = c + band * Mmax + grp_fM[g * nfmax + f];
+            *cp = *cp + (T)s * dv;
+        }
+    }
+}
+
+// add: one thread per (band, covered point), entries in sweep order.
+template <typename T>
+__global__ void lfc_add_kernel(const T* __restrict__ c, T* __restrict__ a, long nG, int Mmax,
+                               long nx, long ncov, const T* __restrict__ A,
+                               const int* __restrict__ cov_G, const int* __restrict__ cov_ptr,
+                               const int2* __restrict__ ent)
+{
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ncov) return;
+    const long i = idx % ncov;
+    const long band = idx / ncov;
+    T* ap = a + band * nG + cov_G[i];
+    const T* cb = c + band * Mmax;
+    T v = *ap;
+    const int e1 = cov_ptr[i + 1];
+    for (int e = cov_ptr[i]; e < e1; e++) {
+        const int2 en = ent[e];
+        const int nm = en.y & 15;
+        const T* Ar = A + en.x;
+        const T* cm = cb + (en.y >> 4);
+        for (int m = 0; m < nm; m++) v = madd<T>(v, Ar[m], cm[m]);
+    }
+    *ap = v;
+}
+
+// ------------------------------------------------------------------ //
+// host-side sweep
+
+struct HostTables {
+    std::vector<int> ptr;            // nG + 1
+    std::vector<int2> ent;
+    std::vector<int> cov_G, cov_ptr;
+    std::map<int, int> key_of_M;     // M -> key
+    std::vector<int> key_M, key_nm;
+    std::vector<std::vector<int>> key_G, key_row;
+};
+
+static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_W,
+                      const int* M_W, int nB, const int* G_B, const int* W_B, long nG,
+                      HostTables& h)
+{
+    struct Vol {
+        long row;    // rows consumed
+        int W;
+    };
+    std::vector<Vol> vol_i;
+    std::vector<long> row_W(nW, 0);
+    std::vector<int> i_W(nW, -1);
+    h.ptr.assign(nG + 1, 0);
+    std::vector<int> count(nG, 0);
+    // first pass: count entries per point
+    {
+        long Ga = 0;
+        int ni = 0;
+        for (int B = 0; B < nB; B++) {
+            long Gb = G_B[B];
+            if (Gb < Ga || Gb > nG) return false;
+            if (Gb > Ga && ni > 0)
+                for (long G = Ga; G < Gb; G++) count[G] += ni;
+            if (W_B[B] >= 0) ni++;
+            else ni--;
+            if (ni < 0) return false;
+            Ga = Gb;
+        }
+        if (ni != 0) return false;
+    }
+    for (long G = 0; G < nG; G++) h.ptr[G + 1] = h.ptr[G] + count[G];
+    h.ent.resize(h.ptr[nG]);
+    std::vector<int> fill(h.ptr.begin(), h.ptr.end() - 1);
+    for (int W = 0; W < nW; W++) {
+        auto it = h.key_of_M.find(M_W[W]);
+        if (it == h.key_of_M.end()) {
+            int k = (int)h.key_M.size();
+            h.key_of_M[M_W[W]] = k;
+            h.key_M.push_back(M_W[W]);
+            h.key_nm.push_back(nm_W[W]);
+        } else if (h.key_nm[it->second] != nm_W[W]) {
+            return false;
+        }
+    }
+    h.key_G.assign(h.key_M.size(), {});
+    h.key_row.assign(h.key_M.size(), {});
+
+    long Ga = 0;
+    for (int B = 0; B < nB; B++) {
+        long Gb = G_B[B];
+        long nGs = Gb - Ga;
+        if (nGs > 0) {
+            for (const Vol& v : vol_i)
+                if (v.row + nGs > ng_W[v.W]) return false;
+            for (long G = Ga; G < Gb; G++)
+                for (const Vol& v : vol_i) {
+                    int nm = nm_W[v.W];
+                    long rowoff = A_off[v.W] + (v.row + (G - Ga)) * nm;
+                    if (rowoff > 0x7fffffffL) return false;
+                    h.ent[fill[G]++] = make_int2((int)rowoff, (M_W[v.W] << 4) | nm);
+                }
+            for (const Vol& v : vol_i) {
+                int k = h.key_of_M[M_W[v.W]];
+                int nm = nm_W[v.W];
+                for (long G = Ga; G < Gb; G++) {
+                    h.key_G[k].push_back((int)G);
+                    h.key_row[k].push_back((int)(A_off[v.W] + (v.row + (G - Ga)) * nm));
+                }
+            }
+            for (Vol& v : vol_i) v.row += nGs;
+        }
+        int Wn = W_B[B];
+        if (Wn >= 0) {
+            if (Wn >= nW || i_W[Wn] >= 0) return false;
+            i_W[Wn] = (int)vol_i.size();
+            vol_i.push_back({row_W[Wn], Wn});
+        } else {
+            int Wold = -1 - Wn;
+            if (Wold >= nW || i_W[Wold] < 0) return false;
+            int iold = i_W[Wold];
+            row_W[Wold] = vol_i[iold].row;
+            vol_i[iold] = vol_i.back();
+            vol_i.pop_back();
+            if (iold < (int)vol_i.size()) i_W[vol_i[iold].W] = iold;
+            i_W[Wold] = -1;
+        }
+        Ga = Gb;
+    }
+    // every volume must have consumed exactly its rows
+    if (!vol_i.empty()) return false;
+    for (int W = 0; W < nW; W++)
+        if (row_W[W] != ng_W[W]) return false;
+    for (long G = 0; G < nG; G++)
+        if (h.ptr[G + 1] > h.ptr[G]) {
+            h.cov_G.push_back((int)G);
+            h.cov_ptr.push_back(h.ptr[G]);
+        }
+    h.cov_ptr.push_back(h.ptr[nG]);
+    return true;
+}
+
+template <typename V>
+static hipError_t upload(V** dst, const V* src, size_t n)
+{
+    *dst = nullptr;
+    if (n == 0) n = 1;   // keep a valid pointer for empty tables
+    hipError_t e = hipMalloc((void**)dst, sizeof(V) * n);
+    if (e != hipSuccess) return e;
+    if (src) return hipMemcpy(*dst, src, sizeof(V) * n, hipMemcpyHostToDevice);
+    return hipMemset(*dst, 0, sizeof(V) * n);
+}
+
+}  // namespace gpd
+
+using namespace gpd;
+
+extern "C" int gpd_lfc_destroy(gpd_lfc* l)
+{
+    if (!l) return GPD_OK;
+    void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
+                    l->key_off, l->key_G, l->key_row, l->grp_off, l->grp_nf, l->grp_fM,
+                    l->grp_G, l->grp_A32};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete l;
+    return GPD_OK;
+}
+
+extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const int* nm_W,
+                              const int* M_W, int nB, const int* G_B, const int* W_B, double dv,
+                              long nG, int Mmax, gpd_lfc** out)
+{
+    GPD_REQUIRE(out, GPD_ERR_ARG, "null output");
+    *out = nullptr;
+    GPD_REQUIRE(nW >= 0 && nB >= 0 && nG > 0 && Mmax >= 0, GPD_ERR_ARG, "bad sizes");
+    GPD_REQUIRE(nW == 0 || (A && ng_W && nm_W && M_W), GPD_ERR_ARG, "null volume arrays");
+    GPD_REQUIRE(nB == 0 || (G_B && W_B), GPD_ERR_ARG, "null boundary arrays");
+    std::vector<long> A_off(nW + 1, 0);
+    for (int W = 0; W < nW; W++) {
+        GPD_REQUIRE(nm_W[W] >= 1 && nm_W[W] <= kNF - 1 && ng_W[W] >= 0, GPD_ERR_ARG,
+                    "volume with unsupported function count");
+        GPD_REQUIRE(M_W[W] >= 0 && M_W[W] + nm_W[W] <= Mmax, GPD_ERR_ARG, "M index out of range");
+        A_off[W + 1] = A_off[W] + (long)ng_W[W] * nm_W[W];
+    }
+    HostTables h;
+    GPD_REQUIRE(run_sweep(nW, A_off.data(), ng_W, nm_W, M_W, nB, G_B, W_B, nG, h), GPD_ERR_ARG,
+                "boundary list inconsistent with the volumes");
+
+    gpd_lfc* l = new (std::nothrow) gpd_lfc();
+    GPD_REQUIRE(l, GPD_ERR_NOMEM, "out of host memory");
+    l->nG = nG;
+    l->Mmax = Mmax;
+    l->dv = dv;
+    l->nvalues = A_off[nW];
+    l->ncov = (long)h.cov_G.size();
+    l->nent = (long)h.ent.size();
+    l->nkeys = (int)h.key_M.size();
+
+    // per-key flat arrays
+    std::vector<long> key_off(l->nkeys + 1, 0);
+    for (int k = 0; k < l->nkeys; k++) key_off[k + 1] = key_off[k] + (long)h.key_G[k].size();
+    std::vector<int> kG, kR;
+    kG.reserve(key_off[l->nkeys]);
+    kR.reserve(key_off[l->nkeys]);
+    for (int k = 0; k < l->nkeys; k++) {
+        kG.insert(kG.end(), h.key_G[k].begin(), h.key_G[k].end());
+        kR.insert(kR.end(), h.key_row[k].begin(), h.key_row[k].end());
+    }
+
+    // groups: keys (in M order) with identical point lists, <= 16 functions
+    std::vector<int> order(l->nkeys);
+    for (int k = 0; k < l->nkeys; k++) order[k] = k;
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return h.key_M[x] < h.key_M[y]; });
+    std::vector<std::vector<int>> groups;
+    std::vector<int> group_nf;
+    for (int k : order) {
+        int found = -1;
+        for (int g = (int)groups.size() - 1; g >= 0 && g >= (int)groups.size() - 8; g--) {
+            int k0 = groups[g][0];
+            if (group_nf[g] + h.key_nm[k] <= kNF && h.key_G[k0] == h.key_G[k]) {
+                found = g;
+                break;
+            }
+        }
+        if (found < 0) {
+            groups.push_back({k});
+            group_nf.push_back(h.key_nm[k]);
+        } else {
+            groups[found].push_back(k);
+            group_nf[found] += h.key_nm[k];
+        }
+    }
+    l->ngroups = (int)groups.size();
+    int nfmax = 1;
+    for (int nf : group_nf) nfmax = std::max(nfmax, nf);
+    l->nfmax = nfmax;
+    std::vector<long> grp_off(l->ngroups + 1, 0);
+    for (int g = 0; g < l->ngroups; g++)
+        grp_off[g + 1] = grp_off[g] + (long)h.key_G[groups[g][0]].size();
+    l->npts_total = grp_off[l->ngroups];
+    std::vector<int> gG(l->npts_total), gfM((size_t)l->ngroups * nfmax, 0);
+    std::vector<float> gA((size_t)l->npts_total * nfmax, 0.f);
+    for (int g = 0; g < l->ngroups; g++) {
+        const std::vector<int>& pts = h.key_G[groups[g][0]];
+        std::copy(pts.begin(), pts.end(), gG.begin() + grp_off[g]);
+        int f = 0;
+        for (int k : groups[g]) {
+            for (int m = 0; m < h.key_nm[k]; m++, f++) {
+                gfM[(size_t)g * nfmax + f] = h.key_M[k] + m;
+                for (size_t j = 0; j < pts.size(); j++)
+                    gA[(grp_off[g] + j) * nfmax + f] = (float)A[h.key_row[k][j] + m];
+            }
+        }
+    }
+    std::vector<float> A32(A, A + l->nvalues);
+
+    hipError_t e = hipSuccess;
+#define UP(dst, src, n) \
+    if (e == hipSuccess) e = upload(&l->dst, src, n)
+    UP(A64, A, l->nvalues);
+    UP(A32, A32.data(), l->nvalues);
+    UP(ptr, h.ptr.data(), h.ptr.size());
+    UP(ent, h.ent.data(), h.ent.size());
+    UP(cov_G, h.cov_G.data(), h.cov_G.size());
+    UP(cov_ptr, h.cov_ptr.data(), h.cov_ptr.size());
+    UP(key_M, h.key_M.data(), h.key_M.size());
+    UP(key_nm, h.key_nm.data(), h.key_nm.size());
+    UP(key_off, key_off.data(), key_off.size());
+    UP(key_G, kG.data(), kG.size());
+    UP(key_row, kR.data(), kR.size());
+    UP(grp_off, grp_off.data(), grp_off.size());
+    UP(grp_nf, group_nf.data(), group_nf.size());
+    UP(grp_fM, gfM.data(), gfM.size());
+    UP(grp_G, gG.data(), gG.size());
+    UP(grp_A32, gA.data(), gA.size());
+#undef UP
+    if (e != hipSuccess) {
+        set_last_error(hipGetErrorString(e));
+        gpd_lfc_destroy(l);
+        return GPD_ERR_HIP;
+    }
+    *out = l;
+    return GPD_OK;
+}
+
+extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a_xG,
+                                 void* d_c_xM, void* stream)
+{
+    GPD_REQUIRE(l, GPD_ERR_ARG, "null lfc");
+    GPD_REQUIRE(nx >= 0, GPD_ERR_ARG, "negative batch");
+    if (nx == 0 || l->nkeys == 0) return GPD_OK;
+    GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64) {
+        long n = nx * l->nkeys;
+        hipLaunchKernelGGL(lfc_integrate_seq_kernel<double>, dim3((n + 63) / 64), dim3(64), 0, st,
+                           (const double*)d_a_xG, (double*)d_c_xM, l->nG, l->Mmax, l->nkeys, nx,
+                           l->dv, l->A64, l->key_M, l->key_nm, l->key_off, l->key_G, l->key_row);
+        return last_error_to_status(hipGetLastError());
+    }
+    if (dtype == GPD_F32) {
+        constexpr int NBB = 4;
+        long nwork = ((nx + NBB - 1) / NBB) * l->ngroups;
+        hipLaunchKernelGGL((lfc_integrate_grp_kernel<float, NBB>), dim3(xcd_grid_size(nwork)),
+                           dim3(256), 0, st, (const float*)d_a_xG, (float*)d_c_xM, l->nG, l->Mmax,
+                           nx, (float)l->dv, l->ngroups, l->nfmax, l->grp_off, l->grp_nf,
+                           l->grp_fM, l->grp_G, l->grp_A32);
+        return last_error_to_status(hipGetLastError());
+    }
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, void* d_a_xG,
+                           void* stream)
+{
+    GPD_REQUIRE(l, GPD_ERR_ARG, "null lfc");
+    GPD_REQUIRE(nx >= 0, GPD_ERR_ARG, "negative batch");
+    if (nx == 0 || l->ncov == 0) return GPD_OK;
+    GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
+    hipStream_t st = (hipStream_t)stream;
+    long n = nx * l->ncov;
+    if (dtype == GPD_F64) {
+        hipLaunchKernelGGL(lfc_add_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           (const double*)d_c_xM, (double*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
+                           l->A64, l->cov_G, l->cov_ptr, l->ent);
+        return last_error_to_status(hipGetLastError());
+    }
+    if (dtype == GPD_F32) {
+        hipLaunchKernelGGL(lfc_add_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, st,
+                           (const float*)d_c_xM, (float*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
+                           l->A32, l->cov_G, l->cov_ptr, l->ent);
+        return last_error_to_status(hipGetLastError());
+    }
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}

@@ -1,0 +1,51 @@
+// Internal interface of the localized-function (projector) kernels.
+#pragma once
+
+#include "common.hpp"
+
+// Device layout built once per geometry from the reference's boundary list.
+//
+// add / fused Hamiltonian: covered points in increasing G; each covered
+// point has entries {row offset of its A row (in values), (M << 4) | nm},
+// listed in the order the reference's boundary sweep visits the active
+// volumes (c/lfc.h:38-87), so FP64 sums reproduce its rounding.
+//
+// integrate: "keys" are the distinct function blocks [M, M + nm); each key
+// has a point list (G, row offset) in the order the sweep visits them for
+// that key.  Keys with identical point lists form a "group" so one pass
+// over the points serves all of their functions.
+struct gpd_lfc {
+    long nG = 0;
+    int Mmax = 0;
+    double dv = 0.0;
+    long nvalues = 0;          // total A values
+    long ncov = 0;             // covered points
+    long nent = 0;             // point entries
+    int nkeys = 0;
+    int ngroups = 0;
+    int nfmax = 0;             // max functions in a group
+    long npts_total = 0;       // sum of group point-list lengths
+    // device arrays
+    double* A64 = nullptr;     // nvalues
+    float* A32 = nullptr;      // nvalues
+    int* ptr = nullptr;        // nG + 1 (dense, for the fused stencil)
+    int2* ent = nullptr;       // nent
+    int* cov_G = nullptr;      // ncov
+    int* cov_ptr = nullptr;    // ncov + 1
+    // integrate, exact order (per key)
+    int* key_M = nullptr;      // nkeys
+    int* key_nm = nullptr;
+    long* key_off = nullptr;   // nkeys + 1 into key_G / key_row
+    int* key_G = nullptr;
+    int* key_row = nullptr;
+    // integrate, grouped (FP32)
+    long* grp_off = nullptr;   // ngroups + 1 into grp_G and rows of grp_A
+    int* grp_nf = nullptr;     // functions per group
+    int* grp_fM = nullptr;     // ngroups * nfmax: M index of each function
+    int* grp_G = nullptr;      // npts_total
+    float* grp_A32 = nullptr;  // npts_total * nfmax (row-major, zero padded)
+};
+
+namespace gpd {
+template <typename T> const T* lfc_values(const gpd_lfc* l);
+}

@@ -1,0 +1,156 @@
+// Multigrid preconditioner (gfx950): the 3-level V-cycle of
+// gpaw/preconditioner.py:34-60 as eight fused launches.
+//
+// The reference runs ~20 separate array passes with numpy temporaries.
+// Here every elementwise step is folded into the transformer or stencil
+// kernel that produces its operand, keeping the reference's per-element
+// arithmetic (scalar factors 4*step and 16*step formed first, then
+// applied; subtractions in the reference's order):
+//   1 r1 = R0(-R),  d1 = (4 step) r1          restrict, scaled copy
+//   2 q1 = T1 d1 - r1                          stencil epilogue
+//   3 r2 = R1(q1),  d2 = (16 step) r2          restrict, scaled copy
+//   4 d2' = d2 - (16 step)(T2 d2 - r2)         stencil epilogue
+//   5 d1 -= I2(d2')                            interpolate epilogue
+//   6 d1' = d1 - (4 step)(T1 d1 - r1)          stencil epilogue
+//   7 d0 = I1(-d1')                            interpolate
+//   8 out = -(d0 + (-step)(T0 d0 - R))         stencil epilogue
+// Step 8's axpy is two roundings here; the reference uses BLAS daxpy,
+// which may fuse it.
+#include <new>
+
+#include "common.hpp"
+#include "fd.hpp"
+#include "transformer.hpp"
+
+struct gpd_precond {
+    gpd_operator* kin[3] = {nullptr, nullptr, nullptr};
+    gpd_transformer* r0 = nullptr;
+    gpd_transformer* r1 = nullptr;
+    gpd_transformer* i2 = nullptr;
+    gpd_transformer* i1 = nullptr;
+    double step = 0.0;
+    long block = 0;
+    long g[3] = {0, 0, 0};
+    void* scratch[2] = {nullptr, nullptr};   // per dtype
+};
+
+namespace gpd {
+
+template <typename T>
+static int precond_apply_typed(gpd_precond* pc, long nb, const T* R, T* out, T* base,
+                               hipStream_t st)
+{
+    const long g1 = pc->g[1], g2 = pc->g[2];
+    T* r1 = base;
+    T* d1 = r1 + nb * g1;
+    T* q1 = d1 + nb * g1;
+    T* r2 = q1 + nb * g1;
+    T* d2 = r2 + nb * g2;
+    T* q2 = d2 + nb * g2;
+    T* d0 = q2 + nb * g2;
+    const double step = pc->step;
+    const T s4 = (T)(4 * step), s16 = (T)(16 * step);
+    int rc;
+    FdEpilogue<T> ep = {};
+
+    rc = transformer_apply_typed<T>(pc->r0, TEPI_SCALED_COPY, true, nb, R, r1, d1, s4, st);
+    if (rc) return rc;
+    ep.aux = r1;
+    rc = fd_apply_typed<T>(pc->kin[1], EPI_SUB, nb, d1, q1, ep, st);
+    if (rc) return rc;
+    rc = transformer_apply_typed<T>(pc->r1, TEPI_SCALED_COPY, false, nb, q1, r2, d2, s16, st);
+    if (rc) return rc;
+    ep.aux = r2;
+    ep.alpha = -s16;
+    ep.neg = 0;
+    rc = fd_apply_typed<T>(pc->kin[2], EPI_SUB_AXPY, nb, d2, q2, ep, st);   // q2 <- d2'
+    if (rc) return rc;
+    rc = transformer_apply_typed<T>(pc->i2, TEPI_SUBTRACT_FROM, false, nb, q2, d1, nullptr, T(0),
+                                    st);
+    if (rc) return rc;
+    ep.aux = r1;
+    ep.alpha = -s4;
+    rc = fd_apply_typed<T>(pc->kin[1], EPI_SUB_AXPY, nb, d1, q1, ep, st);   // q1 <- d1'
+    if (rc) return rc;
+    rc = transformer_apply_typed<T>(pc->i1, TEPI_PLAIN, true, nb, q1, d0, nullptr, T(0), st);
+    if (rc) return rc;
+    ep.aux = R;
+    ep.alpha = (T)(-step);
+    ep.neg = 1;
+    return fd_apply_typed<T>(pc->kin[0], EPI_SUB_AXPY, nb, d0, out, ep, st);
+}
+
+}  // namespace gpd
+
+using namespace gpd;
+
+static long grid_points(const gpd_operator* op)
+{
+    return (long)op->n[0] * op->n[1] * op->n[2];
+}
+
+extern "C" int gpd_precond_create(gpd_operator* kin0, gpd_operator* kin1, gpd_operator* kin2,
+                                  gpd_transformer* r0, gpd_transformer* r1, gpd_transformer* i2,
+                                  gpd_transformer* i1, double step, long block,
+                                  gpd_precond** out)
+{
+    GPD_REQUIRE(out && kin0 && kin1 && kin2 && r0 && r1 && i2 && i1, GPD_ERR_ARG,
+                "null argument");
+    *out = nullptr;
+    GPD_REQUIRE(block >= 1, GPD_ERR_ARG, "block must be >= 1");
+    GPD_REQUIRE(!r0->interpolate && !r1->interpolate && i2->interpolate && i1->interpolate,
+                GPD_ERR_ARG, "wrong transformer kinds");
+    for (int c = 0; c < 3; c++) {
+        bool ok = r0->n_in[c] == kin0->n[c] && r0->n_out[c] == kin1->n[c] &&
+                  r1->n_in[c] == kin1->n[c] && r1->n_out[c] == kin2->n[c] &&
+                  i2->n_in[c] == kin2->n[c] && i2->n_out[c] == kin1->n[c] &&
+                  i1->n_in[c] == kin1->n[c] && i1->n_out[c] == kin0->n[c];
+        GPD_REQUIRE(ok, GPD_ERR_ARG, "grid sizes of the three levels do not match");
+    }
+    gpd_precond* pc = new (std::nothrow) gpd_precond();
+    GPD_REQUIRE(pc, GPD_ERR_NOMEM, "out of host memory");
+    pc->kin[0] = kin0;
+    pc->kin[1] = kin1;
+    pc->kin[2] = kin2;
+    pc->r0 = r0;
+    pc->r1 = r1;
+    pc->i2 = i2;
+    pc->i1 = i1;
+    pc->step = step;
+    pc->block = block;
+    pc->g[0] = grid_points(kin0);
+    pc->g[1] = grid_points(kin1);
+    pc->g[2] = grid_points(kin2);
+    *out = pc;
+    return GPD_OK;
+}
+
+extern "C" int gpd_precond_apply(gpd_precond* pc, int dtype, long nb, const void* d_R,
+                                 void* d_out, void* stream)
+{
+    GPD_REQUIRE(pc, GPD_ERR_ARG, "null preconditioner");
+    GPD_REQUIRE(nb >= 0 && nb <= pc->block, GPD_ERR_ARG, "batch larger than the block size");
+    if (nb == 0) return GPD_OK;
+    GPD_REQUIRE(d_R && d_out && d_R != d_out, GPD_ERR_ARG, "bad arrays");
+    GPD_REQUIRE(dtype == GPD_F64 || dtype == GPD_F32, GPD_ERR_ARG, "unknown dtype");
+    const size_t esize = dtype == GPD_F64 ? sizeof(double) : sizeof(float);
+    if (!pc->scratch[dtype]) {
+        size_t n = (size_t)pc->block * (3 * pc->g[1] + 3 * pc->g[2] + pc->g[0]);
+        GPD_CHECK_HIP(hipMalloc(&pc->scratch[dtype], n * esize));
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64)
+        return precond_apply_typed<double>(pc, nb, (const double*)d_R, (double*)d_out,
+                                           (double*)pc->scratch[0], st);
+    return precond_apply_typed<float>(pc, nb, (const float*)d_R, (float*)d_out,
+                                      (float*)pc->scratch[1], st);
+}
+
+extern "C" int gpd_precond_destroy(gpd_precond* pc)
+{
+    if (!pc) return GPD_OK;
+    for (void* p : pc->scratch)
+        if (p) (void)hipFree(p);
+    delete pc;
+    return GPD_OK;
+}

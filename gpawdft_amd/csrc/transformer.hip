@@ -1,0 +1,401 @@
+// Multigrid restriction and interpolation between grids of 2:1 spacing
+// (gfx950).
+//
+// Behaviour follows _gpaw.Transformer (c/transformers.c:54-185) with the
+// corner-filled halo of c/bc.c (cfd = 0) and the separable 1-D passes of
+// c/bmgs/restrict.c and c/bmgs/interpolate.c: last axis first, then the
+// middle axis, then the first axis, each pass rounded to the element type.
+// One workgroup produces one output tile and keeps both intermediate
+// passes in LDS, so a fine grid is read once (plus the tile halo) and the
+// result written once.  FP64 results are bit-identical to the reference's
+// three-pass code: every intermediate value is computed from the same
+// operands in the same order.
+#include <algorithm>
+#include <new>
+
+#include "common.hpp"
+#include "transformer.hpp"
+
+namespace gpd {
+
+// Lagrange weights, restrict.c:37-57 / interpolate.c:202-215.
+template <int K> struct Weights;
+template <> struct Weights<2> { static constexpr double w[1] = {0.5}; };
+template <> struct Weights<4> { static constexpr double w[2] = {0.5625, -0.0625}; };
+template <> struct Weights<6> { static constexpr double w[3] = {0.58593750, -0.09765625, 0.01171875}; };
+template <> struct Weights<8> {
+    static constexpr double w[4] = {0.59814453125, -0.11962890625, 0.02392578125, -0.00244140625};
+};
+
+// Restriction of one output from a line accessor f(t) = value at offset t
+// from the centre tap: 0.5 * (f0 + w1 (f1 + f-1) + w3 (f3 + f-3) + ...).
+template <int K, typename T, typename F>
+__device__ __forceinline__ T restrict_point(F f)
+{
+    T s = f(0) + (T)Weights<K>::w[0] * (f(1) + f(-1));
+#pragma unroll
+    for (int j = 1; j < K / 2; j++) s = s + (T)Weights<K>::w[j] * (f(2 * j + 1) + f(-2 * j - 1));
+    return (T)0.5 * s;
+}
+
+// Interpolation of the odd point between taps 0 and 1:
+// w1 (f0 + f1) + w3 (f-1 + f2) + ...
+template <int K, typename T, typename F>
+__device__ __forceinline__ T interp_point(F f)
+{
+    T s = (T)Weights<K>::w[0] * (f(0) + f(1));
+#pragma unroll
+    for (int j = 1; j < K / 2; j++) s = s + (T)Weights<K>::w[j] * (f(-j) + f(j + 1));
+    return s;
+}
+
+struct AxisMap {
+    int n_in;     // interior input points
+    int n_out;    // output points
+    int shift;    // interior input index of the centre tap of output 0
+    int pbc;
+    int skip0;    // interpolation: first even point dropped
+};
+
+struct TransformDims {
+    AxisMap ax[3];
+};
+
+// ------------------------------------------------------------------ //
+// restriction: output tile C0 x C1 x C2; input span per axis 2C + 2K - 3
+
+template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG_IN>
+__global__ __launch_bounds__(256) void restrict_kernel(const T* __restrict__ in,
+                                                       T* __restrict__ out,
+                                                       T* __restrict__ out2, T alpha,
+                                                       TransformDims d, int nband)
+{
+    constexpr int H = K - 1;                 // taps on each side
+    constexpr int S0 = 2 * C0 + 2 * H - 1, S1 = 2 * C1 + 2 * H - 1;
+    __shared__ T t2[S0][S1][C2];
+    __shared__ T t1[S0][C1][C2];
+
+    const AxisMap& a0 = d.ax[0];
+    const AxisMap& a1 = d.ax[1];
+    const AxisMap& a2 = d.ax[2];
+    const int nt0 = (a0.n_out + C0 - 1) / C0, nt1 = (a1.n_out + C1 - 1) / C1,
+              nt2 = (a2.n_out + C2 - 1) / C2;
+    const long nwork = (long)nband * nt0 * nt1 * nt2;
+    const long w = xcd_work_index(blockIdx.x, nwork);
+    if (w < 0) return;
+    const int b2 = (int)(w % nt2);
+    long r = w / nt2;
+    const int b1 = (int)(r % nt1);
+    r /= nt1;
+    const int b0 = (int)(r % nt0);
+    const long band = r / nt0;
+    const int o0 = b0 * C0, o1 = b1 * C1, o2 = b2 * C2;
+    // interior input index of span position 0 along each axis
+    const int s0 = 2 * o0 + a0.shift - H, s1 = 2 * o1 + a1.shift - H;
+    const int s2 = 2 * o2 + a2.shift - H;
+
+    const long nin = (long)a0.n_in * a1.n_in * a2.n_in;
+    const T* src = in + band * nin;
+    auto fetch = [&](int i0, int i1, int i2) -> T {
+        int j0 = wrap_index(i0, a0.n_in, a0.pbc);
+        int j1 = wrap_index(i1, a1.n_in, a1.pbc);
+        int j2 = wrap_index(i2, a2.n_in, a2.pbc);
+        if (j0 < 0 || j1 < 0 || j2 < 0) return T(0);
+        T v = src[((long)j0 * a1.n_in + j1) * a2.n_in + j2];
+        return NEG_IN ? -v : v;
+    };
+
+    // pass along axis 2
+    for (int idx = threadIdx.x; idx < S0 * S1 * C2; idx += blockDim.x) {
+        int c2 = idx % C2, q = idx / C2;
+        int p1 = q % S1, p0 = q / S1;
+        int i0 = s0 + p0, i1 = s1 + p1, ic = s2 + H + 2 * c2;
+        t2[p0][p1][c2] = restrict_point<K, T>([&](int t) { return fetch(i0, i1, ic + t); });
+    }
+    __syncthreads();
+    // pass along axis 1
+    for (int idx = threadIdx.x; idx < S0 * C1 * C2; idx += blockDim.x) {
+        int c2 = idx % C2, q = idx / C2;
+        int c1 = q % C1, p0 = q / C1;
+        int pc = H + 2 * c1;
+        t1[p0][c1][c2] = restrict_point<K, T>([&](int t) { return t2[p0][pc + t][c2]; });
+    }
+    __syncthreads();
+    // pass along axis 0 and epilogue
+    const long nout = (long)a0.n_out * a1.n_out * a2.n_out;
+    for (int idx = threadIdx.x; idx < C0 * C1 * C2; idx += blockDim.x) {
+        int c2 = idx % C2, q = idx / C2;
+        int c1 = q % C1, c0 = q / C1;
+        int g0 = o0 + c0, g1 = o1 + c1, g2 = o2 + c2;
+        if (g0 >= a0.n_out || g1 >= a1.n_out || g2 >= a2.n_out) continue;
+        int pc = H + 2 * c0;
+        T v = restrict_point<K, T>([&](int t) { return t1[pc + t][c1][c2]; });
+        long p = band * nout + ((long)g0 * a1.n_out + g1) * a2.n_out + g2;
+        out[p] = v;
+        if (EPI == 1) out2[p] = alpha * v;
+    }
+}
+
+// ------------------------------------------------------------------ //
+// interpolation: output tile F0 x F1 x F2 (even), in unskipped positions
+// u = o + skip0; input span per axis F/2 + K - 1
+
+template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG_IN>
+__global__ __launch_bounds__(256) void interpolate_kernel(const T* __restrict__ in,
+                                                          T* __restrict__ out,
+                                                          TransformDims d, int nband)
+{
+    constexpr int H = K / 2 - 1;             // taps left of tap 0
+    constexpr int S0 = F0 / 2 + K - 1, S1 = F1 / 2 + K - 1;
+    __shared__ T t2[S0][S1][F2];
+    __shared__ T t1[S0][F1][F2];
+
+    const AxisMap& a0 = d.ax[0];
+    const AxisMap& a1 = d.ax[1];
+    const AxisMap& a2 = d.ax[2];
+    // tiles cover unskipped positions [0, n_out + skip0)
+    const int nt0 = (a0.n_out + a0.skip0 + F0 - 1) / F0;
+    const int nt1 = (a1.n_out + a1.skip0 + F1 - 1) / F1;
+    const int nt2 = (a2.n_out + a2.skip0 + F2 - 1) / F2;
+    const long nwork = (long)nband * nt0 * nt1 * nt2;
+    const long w = xcd_work_index(blockIdx.x, nwork);
+    if (w < 0) return;
+    const int b2 = (int)(w % nt2);
+    long r = w / nt2;
+    const int b1 = (int)(r % nt1);
+    r /= nt1;
+    const int b0 = (int)(r % nt0);
+    const long band = r / nt0;
+    const int u0 = b0 * F0, u1 = b1 * F1, u2 = b2 * F2;
+    // input interior index of span position 0: step i = u/2 has tap 0 at
+    // i + shift; the span starts H taps to the left
+    const int s0 = u0 / 2 + a0.shift - H, s1 = u1 / 2 + a1.shift - H;
+    const int s2 = u2 / 2 + a2.shift - H;
+
+    const long nin = (long)a0.n_in * a1.n_in * a2.n_in;
+    const T* src = in + band * nin;
+    auto fetch = [&](int i0, int i1, int i2) -> T {
+        int j0 = wrap_index(i0, a0.n_in, a0.pbc);
+        int j1 = wrap_index(i1, a1.n_in, a1.pbc);
+        int j2 = wrap_index(i2, a2.n_in, a2.pbc);
+        if (j0 < 0 || j1 < 0 || j2 < 0) return T(0);
+        T v = src[((long)j0 * a1.n_in + j1) * a2.n_in + j2];
+        return NEG_IN ? -v : v;
+    };
+
+    // pass along axis 2: position f2 in the tile is u = u2 + f2
+    for (int idx = threadIdx.x; idx < S0 * S1 * F2; idx += blockDim.x) {
+        int f2 = idx % F2, q = idx / F2;
+        int p1 = q % S1, p0 = q / S1;
+        int i0 = s0 + p0, i1 = s1 + p1, ic = s2 + H + f2 / 2;
+        T v;
+        if ((f2 & 1) == 0)
+            v = fetch(i0, i1, ic);
+        else
+            v = interp_point<K, T>([&](int t) { return fetch(i0, i1, ic + t); });
+        t2[p0][p1][f2] = v;
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < S0 * F1 * F2; idx += blockDim.x) {
+        int f2 = idx % F2, q = idx / F2;
+        int f1 = q % F1, p0 = q / F1;
+        int pc = H + f1 / 2;
+        T v;
+        if ((f1 & 1) == 0)
+            v = t2[p0][pc][f2];
+        else
+            v = interp_point<K, T>([&](int t) { return t2[p0][pc + t][f2]; });
+        t1[p0][f1][f2] = v;
+    }
+    __syncthreads();
+    const long nout = (long)a0.n_out * a1.n_out * a2.n_out;
+    for (int idx = threadIdx.x; idx < F0 * F1 * F2; idx += blockDim.x) {
+        int f2 = idx % F2, q = idx / F2;
+        int f1 = q % F1, f0 = q / F1;
+        int g0 = u0 + f0 - a0.skip0, g1 = u1 + f1 - a1.skip0, g2 = u2 + f2 - a2.skip0;
+        if (g0 < 0 || g1 < 0 || g2 < 0 || g0 >= a0.n_out || g1 >= a1.n_out || g2 >= a2.n_out)
+            continue;
+        int pc = H + f0 / 2;
+        T v;
+        if ((f0 & 1) == 0)
+            v = t1[pc][f1][f2];
+        else
+            v = interp_point<K, T>([&](int t) { return t1[pc + t][f1][f2]; });
+        long p = band * nout + ((long)g0 * a1.n_out + g1) * a2.n_out + g2;
+        if (EPI == 2)
+            out[p] = out[p] - v;      // aux -= interpolated value
+        else
+            out[p] = v;
+    }
+}
+
+// ------------------------------------------------------------------ //
+// host side
+
+static TransformDims make_dims(const gpd_transformer* t)
+{
+    TransformDims d;
+    for (int c = 0; c < 3; c++) {
+        AxisMap& m = d.ax[c];
+        m.n_in = t->n_in[c];
+        m.n_out = t->n_out[c];
+        m.pbc = t->pbc[c];
+        m.skip0 = t->interpolate ? t->skip[c][0] : 0;
+        // padded index of the centre tap (restrict: K-1, restrict.c:67) or
+        // of tap 0 (interpolate: K/2-1, interpolate.c:77) minus the left pad
+        m.shift = t->interpolate ? (t->k / 2 - 1) - t->pad[c][0] : (t->k - 1) - t->pad[c][0];
+    }
+    return d;
+}
+
+template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG>
+static int launch_restrict(const gpd_transformer* t, long nb, const T* in, T* out, T* out2,
+                           T alpha, hipStream_t st)
+{
+    TransformDims d = make_dims(t);
+    long tiles = (long)((t->n_out[0] + C0 - 1) / C0) * ((t->n_out[1] + C1 - 1) / C1) *
+                 ((t->n_out[2] + C2 - 1) / C2);
+    long nwork = tiles * nb;
+    hipLaunchKernelGGL((restrict_kernel<T, K, C0, C1, C2, EPI, NEG>), dim3(xcd_grid_size(nwork)),
+                       dim3(256), 0, st, in, out, out2, alpha, d, (int)nb);
+    return last_error_to_status(hipGetLastError());
+}
+
+template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG>
+static int launch_interp(const gpd_transformer* t, long nb, const T* in, T* out, hipStream_t st)
+{
+    TransformDims d = make_dims(t);
+    long tiles = 1;
+    for (int c = 0; c < 3; c++) {
+        int F = c == 0 ? F0 : (c == 1 ? F1 : F2);
+        tiles *= (d.ax[c].n_out + d.ax[c].skip0 + F - 1) / F;
+    }
+    long nwork = tiles * nb;
+    hipLaunchKernelGGL((interpolate_kernel<T, K, F0, F1, F2, EPI, NEG>), dim3(xcd_grid_size(nwork)),
+                       dim3(256), 0, st, in, out, d, (int)nb);
+    return last_error_to_status(hipGetLastError());
+}
+
+template <typename T, int EPI, bool NEG>
+static int dispatch_restrict(const gpd_transformer* t, long nb, const T* in, T* out, T* out2,
+                             T alpha, hipStream_t st)
+{
+    switch (t->k) {
+    case 2: return launch_restrict<T, 2, 4, 8, 16, EPI, NEG>(t, nb, in, out, out2, alpha, st);
+    case 4: return launch_restrict<T, 4, 4, 8, 16, EPI, NEG>(t, nb, in, out, out2, alpha, st);
+    case 6: return launch_restrict<T, 6, 2, 4, 16, EPI, NEG>(t, nb, in, out, out2, alpha, st);
+    case 8: return launch_restrict<T, 8, 2, 4, 16, EPI, NEG>(t, nb, in, out, out2, alpha, st);
+    default: return GPD_ERR_ARG;
+    }
+}
+
+template <typename T, int EPI, bool NEG>
+static int dispatch_interp(const gpd_transformer* t, long nb, const T* in, T* out, hipStream_t st)
+{
+    switch (t->k) {
+    case 2: return launch_interp<T, 2, 8, 8, 32, EPI, NEG>(t, nb, in, out, st);
+    case 4: return launch_interp<T, 4, 8, 8, 32, EPI, NEG>(t, nb, in, out, st);
+    case 6: return launch_interp<T, 6, 4, 8, 32, EPI, NEG>(t, nb, in, out, st);
+    case 8: return launch_interp<T, 8, 4, 8, 32, EPI, NEG>(t, nb, in, out, st);
+    default: return GPD_ERR_ARG;
+    }
+}
+
+template <typename T>
+int transformer_apply_typed(const gpd_transformer* t, int epi, bool neg_in, long nb, const T* in,
+                            T* out, T* out2, T alpha, hipStream_t st)
+{
+    if (!t->interpolate) {
+        if (epi == TEPI_PLAIN)
+            return neg_in ? dispatch_restrict<T, 0, true>(t, nb, in, out, out2, alpha, st)
+                          : dispatch_restrict<T, 0, false>(t, nb, in, out, out2, alpha, st);
+        if (epi == TEPI_SCALED_COPY)
+            return neg_in ? dispatch_restrict<T, 1, true>(t, nb, in, out, out2, alpha, st)
+                          : dispatch_restrict<T, 1, false>(t, nb, in, out, out2, alpha, st);
+        return GPD_ERR_ARG;
+    }
+    if (epi == TEPI_PLAIN)
+        return neg_in ? dispatch_interp<T, 0, true>(t, nb, in, out, st)
+                      : dispatch_interp<T, 0, false>(t, nb, in, out, st);
+    if (epi == TEPI_SUBTRACT_FROM)
+        return neg_in ? dispatch_interp<T, 2, true>(t, nb, in, out, st)
+                      : dispatch_interp<T, 2, false>(t, nb, in, out, st);
+    return GPD_ERR_ARG;
+}
+
+template int transformer_apply_typed<double>(const gpd_transformer*, int, bool, long,
+                                             const double*, double*, double*, double,
+                                             hipStream_t);
+template int transformer_apply_typed<float>(const gpd_transformer*, int, bool, long,
+                                            const float*, float*, float*, float, hipStream_t);
+
+}  // namespace gpd
+
+using namespace gpd;
+
+extern "C" int gpd_transformer_create(const long size_in[3], const long size_out[3], int k,
+                                      const long paddings[6], const long npaddings[6],
+                                      const long skip[6], const long neighbors[6], int real,
+                                      int interpolate, gpd_transformer** out)
+{
+    GPD_REQUIRE(out && size_in && size_out && paddings && npaddings && skip && neighbors,
+                GPD_ERR_ARG, "null argument");
+    *out = nullptr;
+    GPD_REQUIRE(k == 2 || k == 4 || k == 6 || k == 8, GPD_ERR_ARG, "k must be 2, 4, 6 or 8");
+    GPD_REQUIRE(real == 1, GPD_ERR_UNSUPPORTED, "complex transformers are not supported");
+    gpd_transformer* t = new (std::nothrow) gpd_transformer();
+    GPD_REQUIRE(t, GPD_ERR_NOMEM, "out of host memory");
+    t->k = k;
+    t->interpolate = interpolate != 0;
+    for (int c = 0; c < 3; c++) {
+        t->n_in[c] = (int)size_in[c];
+        t->n_out[c] = (int)size_out[c];
+        t->pad[c][0] = (int)paddings[2 * c];
+        t->pad[c][1] = (int)paddings[2 * c + 1];
+        t->skip[c][0] = (int)skip[2 * c];
+        t->skip[c][1] = (int)skip[2 * c + 1];
+        t->pbc[c] = neighbors[2 * c] >= 0 && neighbors[2 * c + 1] >= 0;
+        bool ok = size_in[c] > 0 && size_out[c] > 0 && paddings[2 * c] >= 0 &&
+                  paddings[2 * c + 1] >= 0;
+        // serial periodic halo: each pad is filled from the opposite side
+        if (t->pbc[c])
+            ok = ok && npaddings[2 * c] == paddings[2 * c + 1] &&
+                 npaddings[2 * c + 1] == paddings[2 * c];
+        long m = size_in[c] + paddings[2 * c] + paddings[2 * c + 1];
+        if (!t->interpolate)
+            ok = ok && (m - (2 * k - 3)) / 2 == size_out[c];
+        else
+            ok = ok && 2 * (m - (k - 1) + skip[2 * c + 1]) - skip[2 * c] - skip[2 * c + 1] ==
+                           size_out[c];
+        if (!ok) {
+            delete t;
+            GPD_REQUIRE(false, GPD_ERR_ARG, "inconsistent transformer sizes or paddings");
+        }
+    }
+    *out = t;
+    return GPD_OK;
+}
+
+extern "C" int gpd_transformer_apply(gpd_transformer* t, int dtype, long nin, const void* d_in,
+                                     void* d_out, const void* phases, void* stream)
+{
+    GPD_REQUIRE(t, GPD_ERR_ARG, "null transformer");
+    GPD_REQUIRE(!phases, GPD_ERR_UNSUPPORTED, "Bloch phases are not supported");
+    GPD_REQUIRE(nin >= 0, GPD_ERR_ARG, "negative batch");
+    if (nin == 0) return GPD_OK;
+    GPD_REQUIRE(d_in && d_out && d_in != d_out, GPD_ERR_ARG, "in and out must be distinct");
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64)
+        return transformer_apply_typed<double>(t, TEPI_PLAIN, false, nin, (const double*)d_in,
+                                               (double*)d_out, nullptr, 0.0, st);
+    if (dtype == GPD_F32)
+        return transformer_apply_typed<float>(t, TEPI_PLAIN, false, nin, (const float*)d_in,
+                                              (float*)d_out, nullptr, 0.0f, st);
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_transformer_destroy(gpd_transformer* t)
+{
+    delete t;
+    return GPD_OK;
+}

@@ -1,0 +1,197 @@
+"""Finite-difference operators on device grids.
+
+Drop-in for gpaw/fd_operators.py: `FDOperator(coef_p, offset_pc, gd,
+dtype=float, description=None)` with `.apply(in_xg, out_xg, phase_cd=None)`,
+`.get_diagonal_element()`, `.npoints`, `.description`; `Laplace(gd, scale,
+n, dtype)` / `GUCLaplace` build the Chelikowsky Laplacian coefficients
+exactly as gpaw/fd_operators.py:158-208 does; `Gradient` as :126-148.
+The operator runs as the gfx950 stencil kernels of libgpawdft_amd.so.
+"""
+from math import factorial as fact
+
+import numpy as np
+
+from gpawdft_amd import _lib
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr
+
+# Chelikowsky et al., Phys. Rev. B 50, 11355 (1994): central-difference
+# weights of the second derivative for n = 1..6 (gpaw/fd_operators.py:21-28).
+laplace = [[0],
+           [-2, 1],
+           [-5 / 2, 4 / 3, -1 / 12],
+           [-49 / 18, 3 / 2, -3 / 20, 1 / 90],
+           [-205 / 72, 8 / 5, -1 / 5, 8 / 315, -1 / 560],
+           [-5269 / 1800, 5 / 3, -5 / 21, 5 / 126, -5 / 1008, 1 / 3150],
+           [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925,
+            -1 / 16632]]
+
+_SUPPORTED_DTYPES = (float, np.float64, np.float32)
+
+
+class FDOperator:
+    def __init__(self, coef_p, offset_pc, gd, dtype=float, description=None):
+        offset_pc = np.asarray(offset_pc, int).reshape(-1, 3)
+        # off-axis points need the corner-filled halo (cfd = 0)
+        cfd = not any((o != 0).sum() >= 2 for o in offset_pc)
+        maxoffset_c = offset_pc.max(axis=0)
+        mp = int(maxoffset_c[0])
+        if maxoffset_c[1] != mp or maxoffset_c[2] != mp:
+            mp = int(maxoffset_c.max())
+        n_c = gd.n_c
+        M_c = n_c + 2 * mp
+        stride_c = np.array([M_c[1] * M_c[2], M_c[2], 1])
+        offset_p = np.ascontiguousarray(np.dot(offset_pc, stride_c),
+                                        dtype=np.int64)
+        coef_p = np.ascontiguousarray(coef_p, float)
+        assert coef_p.ndim == 1 and coef_p.shape == offset_p.shape
+        if dtype not in _SUPPORTED_DTYPES:
+            raise NotImplementedError('complex operators are not supported')
+        self.dtype = dtype
+        self.shape = tuple(int(x) for x in n_c)
+        self.mp = mp
+        self.gd = gd
+        self.npoints = len(coef_p)
+        self.coef_p = coef_p
+        self.offset_p = offset_p
+        self.offset_pc = offset_pc
+        self.comm = None
+        self.cfd = cfd
+
+        neighbor = np.ascontiguousarray(gd.neighbor_cd, dtype=np.int64)
+        size = np.ascontiguousarray(n_c, dtype=np.int64)
+        h = _lib.ctypes.c_void_p()
+        check(lib().gpd_operator_create(
+            _lib.ptr_double(coef_p), _lib.ptr_long(offset_p), len(coef_p),
+            _lib.ptr_long(size), mp, _lib.ptr_long(neighbor), 1, int(cfd),
+            _lib.ctypes.byref(h)), 'Operator')
+        self._handle = h
+
+        if description is None:
+            description = '%d point finite-difference stencil' % self.npoints
+        self.description = description
+
+    def __del__(self, _release=_lib.release):
+        _release('gpd_operator_destroy', getattr(self, '_handle', None))
+        self._handle = None
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def __str__(self):
+        return '<' + self.description + '>'
+
+    def apply(self, in_xg, out_xg, phase_cd=None):
+        """out_xg = A in_xg for a batch of grids (last three axes)."""
+        if phase_cd is not None and np.iscomplexobj(np.asarray(phase_cd)):
+            raise NotImplementedError('Bloch phases are not supported')
+        if tuple(in_xg.shape) != tuple(out_xg.shape):
+            raise ValueError('in/out shapes differ')
+        if tuple(in_xg.shape[-3:]) != self.shape:
+            raise ValueError('grid shape %s does not match operator %s'
+                             % (tuple(in_xg.shape[-3:]), self.shape))
+        vin = DeviceView(in_xg)
+        vout = DeviceView(out_xg, copy_in=False)
+        if vin.code != vout.code:
+            raise TypeError('in/out dtypes differ')
+        nin = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
+        check(lib().gpd_operator_apply(self._handle, vin.code, nin, vin.ptr,
+                                       vout.ptr, None, stream_ptr()),
+              'Operator.apply')
+        vout.finish()
+
+    def relax(self, relax_method, f_g, s_g, n, w=None):
+        raise NotImplementedError('relaxation (Poisson solver) is not part '
+                                  'of this build')
+
+    def get_diagonal_element(self):
+        d = _lib.ctypes.c_double()
+        check(lib().gpd_operator_get_diagonal_element(
+            self._handle, _lib.ctypes.byref(d)), 'get_diagonal_element')
+        return d.value
+
+    def get_async_sizes(self):
+        return (0, 1, 1)
+
+
+class Gradient(FDOperator):
+    def __init__(self, gd, v, scale=1.0, n=1, dtype=float):
+        h = (gd.h_cv**2).sum(1)**0.5
+        d = gd.xxxiucell_cv[:, v]
+        A = np.zeros((2 * n + 1, 2 * n + 1))
+        for i, io in enumerate(range(-n, n + 1)):
+            for j in range(2 * n + 1):
+                A[i, j] = io**j / float(fact(j))
+        A[n, 0] = 1.0
+        coefs = np.linalg.inv(A)[1]
+        coefs = np.delete(coefs, len(coefs) // 2)
+        offs = np.delete(np.arange(-n, n + 1), n)
+        coef_p = []
+        offset_pc = []
+        for i in range(3):
+            if abs(d[i]) > 1e-11:
+                coef_p.extend(list(coefs * d[i] / h[i] * scale))
+                offset = np.zeros((2 * n, 3), int)
+                offset[:, i] = offs
+                offset_pc.extend(offset)
+        FDOperator.__init__(self, coef_p, offset_pc, gd, dtype,
+                            'O(h^%d) %s-gradient stencil' % (2 * n, 'xyz'[v]))
+
+
+def Laplace(gd, scale=1.0, n=1, dtype=float):
+    if n == 9:
+        raise NotImplementedError('the FFT Laplacian is not part of this '
+                                  'build')
+    return GUCLaplace(gd, scale, n, dtype)
+
+
+def guc_laplace_stencil(gd, scale=1.0, n=1):
+    """Coefficients and offsets of the O(h^2n) Laplacian for a general
+    (non-orthorhombic) cell, as gpaw/fd_operators.py:158-208 builds them.
+
+    Picks the D shortest lattice directions whose second derivatives
+    reproduce the Laplacian (least squares with zero residual); order:
+    centre, then per direction +1..+n followed by -1..-n.
+    Returns (coefs list, offsets list of 3-vectors, D)."""
+    # the 13 neighbour directions, ordered by length
+    M_ic = np.indices((3, 3, 3)).reshape((3, -3)).T[-13:] - 1
+    u_cv = gd.h_cv / (gd.h_cv**2).sum(1)[:, np.newaxis]**0.5
+    u2_i = (np.dot(M_ic, u_cv)**2).sum(1)
+    i_d = u2_i.argsort()
+
+    m_mv = np.array([(2, 0, 0), (0, 2, 0), (0, 0, 2),
+                     (0, 1, 1), (1, 0, 1), (1, 1, 0)])
+    for D in range(3, 7):
+        h_dv = np.dot(M_ic[i_d[:D]], gd.h_cv)
+        A_md = (h_dv**m_mv[:, np.newaxis, :]).prod(2)
+        a_d, residual, rank, s = np.linalg.lstsq(A_md, [1, 1, 1, 0, 0, 0],
+                                                 rcond=-1)
+        if residual.sum() < 1e-14:
+            if rank != D:
+                raise ValueError(
+                    'You have a weird unit cell!  Try to use the '
+                    'maximally reduced Niggli cell.')
+            break
+
+    a_d *= scale
+    offsets = [(0, 0, 0)]
+    coefs = [laplace[n][0] * a_d.sum()]
+    for d in range(D):
+        M_c = M_ic[i_d[d]]
+        offsets.extend(np.arange(1, n + 1)[:, np.newaxis] * M_c)
+        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
+        offsets.extend(np.arange(-1, -n - 1, -1)[:, np.newaxis] * M_c)
+        coefs.extend(a_d[d] * np.array(laplace[n][1:]))
+    return coefs, offsets, D
+
+
+class GUCLaplace(FDOperator):
+    def __init__(self, gd, scale=1.0, n=1, dtype=float):
+        """Laplacian for general (non-orthorhombic) cells, O(h^2n);
+        scale=-0.5 gives the kinetic-energy operator."""
+        coefs, offsets, D = guc_laplace_stencil(gd, scale, n)
+        FDOperator.__init__(self, coefs, offsets, gd, dtype)
+        self.description = (
+            '%d*%d+1=%d point O(h^%d) finite-difference Laplacian' %
+            ((self.npoints - 1) // n, n, self.npoints, 2 * n))

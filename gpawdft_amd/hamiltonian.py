@@ -1,0 +1,198 @@
+"""The H psi part of the PAW Hamiltonian on device grids.
+
+Mirrors the pieces of the reference this path needs:
+* `unpack` / `pack2` (gpaw/utilities/__init__.py:144-197);
+* `Hamiltonian.apply_local_potential` and `Hamiltonian.apply`
+  (gpaw/hamiltonian.py:397-460) for one spin channel set;
+* `FDWaveFunctions.apply_pseudo_hamiltonian` / `make_preconditioner`
+  (gpaw/wavefunctions/fd.py:40-83);
+* `KPoint`, the per-k-point state the solvers read (gpaw/kpoint.py).
+
+`Hamiltonian.apply_fused` is the fast path: projections, P dH and one
+stencil launch whose epilogue adds v psi and the projector terms.
+"""
+import numpy as np
+
+from gpawdft_amd import _lib
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr, to_device, torch
+from gpawdft_amd.fd_operators import Laplace
+from gpawdft_amd.preconditioner import Preconditioner
+
+
+def unpack(M):
+    """Packed upper triangle (row-major) -> symmetric matrix
+    (c/utilities.c:535-552)."""
+    M = np.asarray(M)
+    n = int(np.sqrt(0.25 + 2.0 * len(M)))
+    M2 = np.zeros((n, n), M.dtype)
+    p = 0
+    for r in range(n):
+        for c in range(r, n):
+            M2[r, c] = M2[c, r] = M[p]
+            p += 1
+    return M2
+
+
+def pack2(M2, tolerance=1e-10):
+    """Symmetric matrix -> packed upper triangle, averaging off-diagonals."""
+    n = len(M2)
+    M = np.zeros(n * (n + 1) // 2, M2.dtype)
+    p = 0
+    for r in range(n):
+        M[p] = M2[r, r]
+        p += 1
+        for c in range(r + 1, n):
+            M[p] = (M2[r, c] + np.conjugate(M2[c, r])) / 2.
+            assert abs(M2[r, c] - np.conjugate(M2[c, r])) < tolerance
+            p += 1
+    return M
+
+
+class KPoint:
+    """Gamma-point, one spin: the state the eigensolvers work on."""
+
+    def __init__(self, psit_nG=None, s=0, weight=1.0, f_n=None):
+        self.s = s
+        self.q = -1
+        self.k = 0
+        self.weight = weight
+        self.phase_cd = None
+        self.psit_nG = psit_nG
+        self.P_ani = None
+        self.eps_n = None
+        self.f_n = f_n
+
+
+class Hamiltonian:
+    """Local potential vt_sG (host or device) and packed dH_asp."""
+
+    def __init__(self, gd, vt_sG, dH_asp, dO_aii=None):
+        self.gd = gd
+        self.vt_sG = vt_sG
+        self.dH_asp = dH_asp
+        self.dO_aii = dO_aii or {}
+        self._dev = {}
+
+    def _vt(self, s, dtype, device):
+        key = ('vt', s, dtype)
+        if key not in self._dev:
+            self._dev[key] = to_device(self.vt_sG[s], dtype).to(device)
+        return self._dev[key]
+
+    def _dH_tables(self, pt, s, dtype, device):
+        """Device tables for gpd_projector_coefs: M_a, ni_a, offsets and
+        the unpacked dH_ii blocks."""
+        key = ('dH', id(pt), s, dtype)
+        if key not in self._dev:
+            atoms = pt.atom_indices
+            M_a, ni_a, off, blocks = [], [], [], []
+            o = 0
+            for a in atoms:
+                dH_ii = unpack(np.asarray(self.dH_asp[a][s], float))
+                if len(dH_ii) != pt.get_function_count(a):
+                    raise ValueError('dH of atom %d is %dx%d, the atom has %d '
+                                     'projectors' % (a, len(dH_ii),
+                                                     len(dH_ii),
+                                                     pt.get_function_count(a)))
+                M_a.append(pt.M_a[a])
+                ni_a.append(len(dH_ii))
+                off.append(o)
+                o += dH_ii.size
+                blocks.append(dH_ii.ravel())
+            dev = lambda x, dt: to_device(np.asarray(x, dt)).to(device)
+            self._dev[key] = (len(atoms), dev(M_a, np.intc),
+                              dev(ni_a, np.intc), dev(off, np.intc),
+                              dev(np.concatenate(blocks) if blocks
+                                  else np.zeros(1), float).to(dtype))
+        return self._dev[key]
+
+    def apply_local_potential(self, psit_nG, Htpsit_nG, s=0):
+        """Htpsit_nG += psit_nG * vt_G (gpaw/hamiltonian.py:397-424)."""
+        vin = DeviceView(psit_nG)
+        vout = DeviceView(Htpsit_nG)
+        vt = self._vt(s, vin.t.dtype, vin.t.device)
+        nG = int(np.prod(psit_nG.shape[-3:]))
+        nb = int(np.prod(psit_nG.shape[:-3], dtype=np.int64))
+        check(lib().gpd_add_local_potential(vin.code, nb, nG, vin.ptr,
+                                            vt.data_ptr(), vout.ptr,
+                                            stream_ptr()),
+              'apply_local_potential')
+        vout.finish()
+
+    def apply(self, a_xG, b_xG, wfs, kpt, calculate_P_ani=True):
+        """b_xG = H a_xG (kinetic + local + non-local) as the reference's
+        sequence of operator calls (gpaw/hamiltonian.py:426-460)."""
+        va = DeviceView(a_xG)
+        vb = DeviceView(b_xG, copy_in=False)
+        wfs.kin.apply(va.t, vb.t, kpt.phase_cd)
+        self.apply_local_potential(va.t, vb.t, kpt.s)
+        pt = wfs.pt
+        nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+        P_xM = torch.zeros((nx, pt.Mmax), dtype=va.t.dtype,
+                           device=va.t.device)
+        if calculate_P_ani:
+            pt.integrate_xM(va, P_xM)
+        else:
+            P_xM = pt._pack(kpt.P_ani, tuple(a_xG.shape[:-3]), va.t)
+        c_xM = self.projector_coefs(pt, P_xM, kpt.s)
+        pt.add_xM(c_xM, vb.t)
+        vb.finish()
+
+    def projector_coefs(self, pt, P_xM, s=0, out=None):
+        """c_xM = P_xM dH (block diagonal over atoms), on the device."""
+        natoms, M_a, ni_a, off, dH = self._dH_tables(pt, s, P_xM.dtype,
+                                                     P_xM.device)
+        nx = P_xM.shape[0]
+        c_xM = torch.empty_like(P_xM) if out is None else out
+        code = DeviceView(P_xM).code
+        check(lib().gpd_projector_coefs(
+            code, nx, int(pt.Mmax), natoms, M_a.data_ptr(), ni_a.data_ptr(),
+            off.data_ptr(), dH.data_ptr(), P_xM.data_ptr(), c_xM.data_ptr(),
+            stream_ptr()), 'projector_coefs')
+        return c_xM
+
+    def apply_fused(self, psit_nG, Htpsit_nG, kin, pt=None, s=0,
+                    P_xM=None, c_xM=None):
+        """Htpsit = T psit + v psit + sum_a p_a P_a dH_a in three launches:
+        projections, P dH, and the stencil with the fused epilogue.
+        Returns P_xM (the projections) or None without projectors."""
+        nb = int(np.prod(psit_nG.shape[:-3], dtype=np.int64))
+        vin = DeviceView(psit_nG)
+        vout = DeviceView(Htpsit_nG, copy_in=False)
+        vt = self._vt(s, vin.t.dtype, vin.t.device)
+        lfc_h = None
+        c_ptr = None
+        if pt is not None and pt.Mmax:
+            if P_xM is None:
+                P_xM = torch.zeros((nb, pt.Mmax), dtype=vin.t.dtype,
+                                   device=vin.t.device)
+            else:
+                P_xM.zero_()
+            pt.integrate_xM(vin, P_xM)
+            c_xM = self.projector_coefs(pt, P_xM, s, c_xM)
+            lfc_h = pt.handle
+            c_ptr = c_xM.data_ptr()
+        check(lib().gpd_hamiltonian_apply(kin.handle, lfc_h, vin.code, nb,
+                                          vin.ptr, vt.data_ptr(), c_ptr,
+                                          vout.ptr, stream_ptr()),
+              'hamiltonian_apply')
+        vout.finish()
+        return P_xM
+
+
+class FDWaveFunctions:
+    """The grid-mode wave-function object's operator set."""
+
+    def __init__(self, gd, pt=None, stencil=3, dtype=float):
+        self.gd = gd
+        self.dtype = dtype
+        self.kin = Laplace(gd, -0.5, stencil, dtype)
+        self.pt = pt
+
+    def make_preconditioner(self, block=1):
+        return Preconditioner(self.gd, self.kin, self.dtype, block)
+
+    def apply_pseudo_hamiltonian(self, kpt, ham, psit_xG, Htpsit_xG):
+        self.kin.apply(psit_xG, Htpsit_xG, kpt.phase_cd)
+        ham.apply_local_potential(psit_xG, Htpsit_xG, kpt.s)

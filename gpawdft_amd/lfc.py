@@ -1,0 +1,307 @@
+"""Localized functions (PAW projectors) on device grids.
+
+Drop-in for gpaw/lfc.py's NewLocalizedFunctionsCollection (real, serial):
+`LFC(gd, spline_aj, kd=None, cut=False, dtype=float, integral=None,
+forces=False)` with `.set_positions(spos_ac)`, `.dict(shape, derivative,
+zero)`, `.add(a_xG, c_axi=1.0, q=-1)`, `.integrate(a_xG, c_axi, q=-1)`.
+
+Set-up (per geometry) follows gpaw/lfc.py:77-115 and :276-350: each sphere
+is split into periodic-image boxes, each (spline, box) tabulated on the
+grid by gpd_spline_to_grid (a volume W), and the boundary list (G_B, W_B)
+stable-sorted by G.  The device object built from it runs integrate/add
+on the GPU.
+"""
+import numpy as np
+
+from gpawdft_amd import _lib
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr, torch, zeros
+from gpawdft_amd.grid_descriptor import GridBoundsError
+
+
+def spline_to_grid(spline, gd, beg_c, end_c, spos_c):
+    """Tabulate spline(r) r^l Y_lm on the box [beg_c, end_c) around the
+    scaled position spos_c (c/lfc.c:785-875).  Returns (A_gm, G_b)."""
+    pos_v = np.ascontiguousarray(np.dot(spos_c, gd.cell_cv), float)
+    beg = np.ascontiguousarray(beg_c, np.int64)
+    end = np.ascontiguousarray(end_c, np.int64)
+    nbox = int(np.prod(end - beg))
+    nm = 2 * spline.get_angular_momentum_number() + 1
+    A = np.empty((max(nbox, 1), nm))
+    G = np.empty(2 * max(int((end[0] - beg[0]) * (end[1] - beg[1])), 1),
+                 np.intc)
+    h_cv = np.ascontiguousarray(gd.h_cv, float)
+    n_c = np.ascontiguousarray(gd.n_c, np.int64)
+    corner = np.ascontiguousarray(gd.beg_c, np.int64)
+    ng = _lib.ctypes.c_long()
+    nB = _lib.ctypes.c_long()
+    check(lib().gpd_spline_to_grid(
+        spline.handle, _lib.ptr_long(beg), _lib.ptr_long(end),
+        _lib.ptr_double(pos_v), _lib.ptr_double(h_cv), _lib.ptr_long(n_c),
+        _lib.ptr_long(corner), _lib.ptr_double(A), _lib.ptr_int(G),
+        _lib.ctypes.byref(ng), _lib.ctypes.byref(nB)), 'spline_to_grid')
+    return A[:ng.value].copy(), G[:nB.value].copy()
+
+
+class Sphere:
+    def __init__(self, spline_j):
+        self.spline_j = spline_j
+        self.spos_c = None
+        self.rank = None
+        self.ranks = None
+        self.Mmax = None
+        self.A_wgm = None
+        self.G_wb = None
+        self.M_w = None
+        self.sdisp_wc = None
+
+    def set_position(self, spos_c, gd, cut):
+        if self.spos_c is not None and not (self.spos_c - spos_c).any():
+            return False
+        self.A_wgm = []
+        self.G_wb = []
+        self.M_w = []
+        self.sdisp_wc = []
+        ng = 0
+        M = 0
+        for spline in self.spline_j:
+            rcut = spline.get_cutoff()
+            l = spline.get_angular_momentum_number()
+            for beg_c, end_c, sdisp_c in gd.get_boxes(spos_c, rcut, cut):
+                A_gm, G_b = spline_to_grid(spline, gd, beg_c, end_c,
+                                           spos_c - sdisp_c)
+                if len(G_b) > 0:
+                    self.A_wgm.append(A_gm)
+                    self.G_wb.append(G_b)
+                    self.M_w.append(M)
+                    self.sdisp_wc.append(sdisp_c)
+                    ng += A_gm.shape[0]
+            M += 2 * l + 1
+        self.Mmax = M
+        self.rank = gd.get_rank_from_position(spos_c)
+        if ng == 0:
+            self.ranks = None
+            self.A_wgm = None
+            self.G_wb = None
+            self.M_w = None
+            self.sdisp_wc = None
+        self.spos_c = np.array(spos_c, float)
+        return True
+
+    def get_function_count(self):
+        return sum([2 * spline.get_angular_momentum_number() + 1
+                    for spline in self.spline_j])
+
+
+class NewLocalizedFunctionsCollection:
+    def __init__(self, gd, spline_aj, kd=None, cut=False, dtype=float,
+                 integral=None, forces=None):
+        if kd is not None and not getattr(kd, 'gamma', True):
+            raise NotImplementedError('k-points (Bloch phases) are not '
+                                      'supported')
+        if integral is not None:
+            raise NotImplementedError('normalised LFCs are not part of this '
+                                      'build')
+        self.gd = gd
+        self.sphere_a = [Sphere(spline_j) for spline_j in spline_aj]
+        self.cut = cut
+        self.dtype = dtype
+        self.Mmax = None
+        self.gamma = True
+        self.use_global_indices = False
+        self.my_atom_indices = None
+        self._handle = None
+
+    def __del__(self):
+        self._destroy()
+
+    def _destroy(self, _release=_lib.release):
+        _release('gpd_lfc_destroy', getattr(self, '_handle', None))
+        self._handle = None
+
+    def get_function_count(self, a):
+        return self.sphere_a[a].get_function_count()
+
+    def set_positions(self, spos_ac):
+        assert len(spos_ac) == len(self.sphere_a)
+        spos_ac = np.asarray(spos_ac, float)
+        movement = False
+        for a, (spos_c, sphere) in enumerate(zip(spos_ac, self.sphere_a)):
+            try:
+                movement |= sphere.set_position(spos_c, self.gd, self.cut)
+            except GridBoundsError as e:
+                e.args = ['Atom %d too close to edge: %s' % (a, str(e))]
+                raise
+        if movement or self.my_atom_indices is None:
+            self._update(spos_ac)
+
+    def _update(self, spos_ac):
+        nB = 0
+        nW = 0
+        self.my_atom_indices = []
+        self.atom_indices = []
+        M = 0
+        self.M_a = []
+        for a, sphere in enumerate(self.sphere_a):
+            self.M_a.append(M)
+            if sphere.rank == self.gd.comm.rank:
+                self.my_atom_indices.append(a)
+            if sphere.G_wb:
+                nB += sum([len(G_b) for G_b in sphere.G_wb])
+                nW += len(sphere.G_wb)
+                self.atom_indices.append(a)
+                M += sphere.Mmax
+        self.Mmax = M
+
+        self.M_W = np.empty(nW, np.intc)
+        self.G_B = np.empty(nB, np.intc)
+        self.W_B = np.empty(nB, np.intc)
+        self.A_Wgm = []
+        sdisp_Wc = np.empty((nW, 3), int)
+        self.pos_Wv = np.empty((nW, 3))
+        B1 = 0
+        W = 0
+        for a in self.atom_indices:
+            sphere = self.sphere_a[a]
+            self.A_Wgm.extend(sphere.A_wgm)
+            nw = len(sphere.M_w)
+            self.M_W[W:W + nw] = self.M_a[a] + np.array(sphere.M_w)
+            sdisp_Wc[W:W + nw] = sphere.sdisp_wc
+            self.pos_Wv[W:W + nw] = np.dot(spos_ac[a] -
+                                           np.array(sphere.sdisp_wc),
+                                           self.gd.cell_cv)
+            for G_b in sphere.G_wb:
+                B2 = B1 + len(G_b)
+                self.G_B[B1:B2] = G_b
+                self.W_B[B1:B2:2] = W
+                self.W_B[B1 + 1:B2 + 1:2] = -W - 1
+                B1 = B2
+                W += 1
+        assert B1 == nB
+
+        indices = np.argsort(self.G_B, kind='mergesort')
+        self.G_B = np.ascontiguousarray(self.G_B[indices])
+        self.W_B = np.ascontiguousarray(self.W_B[indices])
+
+        self._destroy()
+        for a in self.atom_indices:
+            self.sphere_a[a].ranks = np.array([], int)
+        return sdisp_Wc
+
+    def host_tables(self):
+        """(A concatenated, ng_W, nm_W) of the current volumes."""
+        ng_W = np.array([A.shape[0] for A in self.A_Wgm], np.intc)
+        nm_W = np.array([A.shape[1] for A in self.A_Wgm], np.intc)
+        A = (np.ascontiguousarray(np.concatenate([A.ravel()
+                                                  for A in self.A_Wgm]))
+             if self.A_Wgm else np.zeros(1))
+        return A, ng_W, nm_W
+
+    def _create_device_object(self):
+        A, ng_W, nm_W = self.host_tables()
+        h = _lib.ctypes.c_void_p()
+        nG = int(np.prod(self.gd.n_c))
+        check(lib().gpd_lfc_create(
+            len(self.A_Wgm), _lib.ptr_double(A), _lib.ptr_int(ng_W),
+            _lib.ptr_int(nm_W), _lib.ptr_int(self.M_W), len(self.G_B),
+            _lib.ptr_int(self.G_B), _lib.ptr_int(self.W_B),
+            float(self.gd.dv), nG, int(self.Mmax), _lib.ctypes.byref(h)),
+            'LFC')
+        self._handle = h
+
+    @property
+    def handle(self):
+        """Device object, built on first use after set_positions."""
+        if self._handle is None:
+            self._create_device_object()
+        return self._handle
+
+    def M_range(self, a):
+        return self.M_a[a], self.M_a[a] + self.sphere_a[a].Mmax
+
+    def dict(self, shape=(), derivative=False, zero=False, device=None):
+        if isinstance(shape, int):
+            shape = (shape,)
+        if derivative:
+            raise NotImplementedError('derivatives (forces) are not part of '
+                                      'this build')
+        c_axi = {}
+        for a in self.my_atom_indices:
+            ni = self.get_function_count(a)
+            c_axi[a] = zeros(tuple(shape) + (ni,), self.dtype, device)
+        return c_axi
+
+    def _pack(self, c_axi, xshape, like):
+        """c_axi dict -> contiguous (nx, Mmax) device tensor."""
+        nx = int(np.prod(xshape, dtype=np.int64))
+        c_xM = torch.zeros((nx, self.Mmax), dtype=like.dtype,
+                           device=like.device)
+        for a in self.atom_indices:
+            M1, M2 = self.M_range(a)
+            c = c_axi[a]
+            if not isinstance(c, torch.Tensor):
+                c = torch.from_numpy(np.asarray(c))
+            c_xM[:, M1:M2] = c.reshape(nx, M2 - M1).to(c_xM.device,
+                                                         c_xM.dtype)
+        return c_xM
+
+    def add(self, a_xG, c_axi=1.0, q=-1):
+        """a_xG += sum_{a,i} c_axi Phi^a_i (G)."""
+        assert not self.use_global_indices
+        va = DeviceView(a_xG)
+        xshape = tuple(a_xG.shape[:-3])
+        nx = int(np.prod(xshape, dtype=np.int64))
+        if isinstance(c_axi, float):
+            assert a_xG.ndim == 3
+            c_xM = torch.full((1, self.Mmax), c_axi, dtype=va.t.dtype,
+                              device=va.t.device)
+        else:
+            c_xM = self._pack(c_axi, xshape, va.t)
+        check(lib().gpd_lfc_add(self.handle, va.code, nx, c_xM.data_ptr(),
+                                va.ptr, stream_ptr()), 'LFC.add')
+        va.finish()
+
+    def integrate(self, a_xG, c_axi, q=-1):
+        """c_axi = integral of a_x(G) Phi^a_i(G) over the grid."""
+        assert not self.use_global_indices
+        va = DeviceView(a_xG)
+        xshape = tuple(a_xG.shape[:-3])
+        nx = int(np.prod(xshape, dtype=np.int64))
+        c_xM = torch.zeros((nx, self.Mmax), dtype=va.t.dtype,
+                           device=va.t.device)
+        self.integrate_xM(va, c_xM)
+        for a in self.atom_indices:
+            if a not in c_axi:
+                continue
+            M1, M2 = self.M_range(a)
+            dst = c_axi[a]
+            src = c_xM[:, M1:M2].reshape(xshape + (M2 - M1,))
+            if isinstance(dst, np.ndarray):
+                dst[...] = src.cpu().numpy()
+            else:
+                dst.copy_(src)
+
+    def integrate_xM(self, view_or_tensor, c_xM):
+        """Low-level: c_xM (nx, Mmax device tensor) += integrals."""
+        va = (view_or_tensor if isinstance(view_or_tensor, DeviceView)
+              else DeviceView(view_or_tensor))
+        nx = int(np.prod(va.t.shape[:-3], dtype=np.int64))
+        check(lib().gpd_lfc_integrate(self.handle, va.code, nx, va.ptr,
+                                      c_xM.data_ptr(), stream_ptr()),
+              'LFC.integrate')
+
+    def add_xM(self, c_xM, a_xG):
+        """Low-level: a_xG += sum_M c_xM A_GM (device tensors)."""
+        va = DeviceView(a_xG)
+        nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+        check(lib().gpd_lfc_add(self.handle, va.code, nx, c_xM.data_ptr(),
+                                va.ptr, stream_ptr()), 'LFC.add')
+
+
+LocalizedFunctionsCollection = NewLocalizedFunctionsCollection
+
+
+def LFC(gd, spline_aj, kd=None, cut=False, dtype=float, integral=None,
+        forces=False):
+    return NewLocalizedFunctionsCollection(gd, spline_aj, kd, cut, dtype,
+                                           integral, forces)

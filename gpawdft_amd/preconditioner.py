@@ -1,0 +1,77 @@
+"""Multigrid preconditioner.
+
+Drop-in for gpaw/preconditioner.py: `Preconditioner(gd0, kin0, dtype=float,
+block=1)`, `__call__(residuals, kpt, ekin=None) -> d0`,
+`calculate_kinetic_energy(psit_xG, kpt) -> None`.  Same V-cycle (nn=1
+transformers, 7-point Laplacians on the coarse grids, step =
+0.66666666 / diag(kin0)), executed as eight fused gfx950 launches that
+keep the reference's per-element arithmetic.
+"""
+import numpy as np
+
+from gpawdft_amd import _lib
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr
+from gpawdft_amd.fd_operators import Laplace
+from gpawdft_amd.transformers import Transformer
+
+
+class Preconditioner:
+    def __init__(self, gd0, kin0, dtype=float, block=1):
+        gd1 = gd0.coarsen()
+        gd2 = gd1.coarsen()
+        self.gd0 = gd0
+        self.kin0 = kin0
+        self.kin1 = Laplace(gd1, -0.5, 1, dtype)
+        self.kin2 = Laplace(gd2, -0.5, 1, dtype)
+        self.step = 0.66666666 / kin0.get_diagonal_element()
+        self.block = block
+        self.dtype = dtype
+
+        self.restrictor_object0 = Transformer(gd0, gd1, 1, dtype)
+        self.restrictor_object1 = Transformer(gd1, gd2, 1, dtype)
+        self.interpolator_object2 = Transformer(gd2, gd1, 1, dtype)
+        self.interpolator_object1 = Transformer(gd1, gd0, 1, dtype)
+        self.restrictor0 = self.restrictor_object0.apply
+        self.restrictor1 = self.restrictor_object1.apply
+        self.interpolator2 = self.interpolator_object2.apply
+        self.interpolator1 = self.interpolator_object1.apply
+
+        h = _lib.ctypes.c_void_p()
+        check(lib().gpd_precond_create(
+            kin0.handle, self.kin1.handle, self.kin2.handle,
+            self.restrictor_object0.handle, self.restrictor_object1.handle,
+            self.interpolator_object2.handle,
+            self.interpolator_object1.handle, self.step, int(block),
+            _lib.ctypes.byref(h)), 'Preconditioner')
+        self._handle = h
+
+    def __del__(self, _release=_lib.release):
+        _release('gpd_precond_destroy', getattr(self, '_handle', None))
+        self._handle = None
+
+    def calculate_kinetic_energy(self, psit_xG, kpt):
+        return None
+
+    def __call__(self, residuals, kpt=None, ekin=None, out=None):
+        """Return the preconditioned correction for `residuals`
+        (nb <= block grids).  `out` may be given to avoid an allocation."""
+        if residuals.ndim == 3:
+            return self(residuals[None], kpt, ekin, None if out is None
+                        else out[None])[0]
+        nb = residuals.shape[0]
+        if nb > self.block:
+            raise ValueError('%d residuals exceed block=%d' % (nb, self.block))
+        if out is None:
+            if isinstance(residuals, np.ndarray):
+                out = np.empty_like(residuals)
+            else:
+                out = self.gd0.empty(nb, dtype=residuals.dtype,
+                                     device=residuals.device)
+        vin = DeviceView(residuals)
+        vout = DeviceView(out, copy_in=False)
+        check(lib().gpd_precond_apply(self._handle, vin.code, nb, vin.ptr,
+                                      vout.ptr, stream_ptr()),
+              'Preconditioner')
+        vout.finish()
+        return out

@@ -1,0 +1,191 @@
+/*
+ * gpawdft_amd -- C ABI of the MI355X (gfx950) real-space grid Hamiltonian
+ * path.  Plain pointers and sizes only.  Array arguments named d_* are
+ * device pointers (HIP global memory); everything else is host memory.
+ * `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Grids are C-contiguous (..., n0, n1, n2) with n2 the contiguous axis;
+ * a batch of nb grids is nb consecutive grids.  dtype selects the element
+ * type of the device arrays: GPD_F64 (bit-compatible with the reference's
+ * FP64 C code) or GPD_F32.
+ *
+ * Every function returns an int status: 0 on success, < 0 for a bad
+ * argument or a HIP failure (gpd_last_error() gives the message), > 0 for
+ * a LAPACK-style info from the dense FP64 routines.  Calls are stream
+ * ordered: they return once the work is queued.
+ */
+#ifndef GPAWDFT_AMD_H
+#define GPAWDFT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GPD_OK = 0,
+    GPD_ERR_ARG = -1,
+    GPD_ERR_HIP = -2,
+    GPD_ERR_UNSUPPORTED = -3,
+    GPD_ERR_NOMEM = -4
+};
+
+enum { GPD_F64 = 0, GPD_F32 = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gpd_last_error(void);
+
+/* Library version string. */
+const char* gpd_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Finite-difference operator.
+ * Replaces _gpaw.Operator(coefs, offsets, size, range, neighbors, real,
+ * comm, cfd) (c/operators.c:462-495): `offsets` are flat offsets with the
+ * padded strides [(n1+2r)(n2+2r), n2+2r, 1] that gpaw/fd_operators.py:43-52
+ * builds; neighbors[c][d] >= 0 marks a periodic axis (a single rank is its
+ * own neighbour), -1 a zero boundary.  Only real operators (real == 1) are
+ * supported; complex Bloch phases are rejected.                        */
+typedef struct gpd_operator gpd_operator;
+
+int gpd_operator_create(const double* coefs, const long* offsets, int ncoefs,
+                        const long size[3], int range,
+                        const long neighbors[6], int real, int cfd,
+                        gpd_operator** out);
+
+/* Operator.apply(in, out, phases) (c/operators.c:316-405): out = A in for
+ * nin grids.  phases must be NULL.                                    */
+int gpd_operator_apply(gpd_operator* op, int dtype, long nin,
+                       const void* d_in, void* d_out, const void* phases,
+                       void* stream);
+
+/* Operator.get_diagonal_element() (c/operators.c:408-421). */
+int gpd_operator_get_diagonal_element(const gpd_operator* op, double* diag);
+
+int gpd_operator_destroy(gpd_operator* op);
+
+/* ------------------------------------------------------------------ */
+/* Restriction / interpolation between grids of 2:1 spacing.
+ * Replaces _gpaw.Transformer(size_in, size_out, k, paddings, npaddings,
+ * skip, neighbors, real, comm, interpolate) (c/transformers.c:219-262);
+ * paddings/npaddings/skip/neighbors are [3][2] as computed by
+ * gpaw/transformers.py:20-76.  k = 2, 4, 6 or 8.                       */
+typedef struct gpd_transformer gpd_transformer;
+
+int gpd_transformer_create(const long size_in[3], const long size_out[3],
+                           int k, const long paddings[6],
+                           const long npaddings[6], const long skip[6],
+                           const long neighbors[6], int real, int interpolate,
+                           gpd_transformer** out);
+
+/* Transformer.apply(in, out, phases) (c/transformers.c:127-185). */
+int gpd_transformer_apply(gpd_transformer* t, int dtype, long nin,
+                          const void* d_in, void* d_out, const void* phases,
+                          void* stream);
+
+int gpd_transformer_destroy(gpd_transformer* t);
+
+/* ------------------------------------------------------------------ */
+/* Radial cubic spline (FP64, host).  Replaces _gpaw.Spline(l, rcut, f_g)
+ * (c/spline.c:121-135, c/bmgs/spline.c:11-39).  f[npts-1] is used as
+ * given (gpaw/spline.py:25 sets it to zero before the call).          */
+typedef struct gpd_spline gpd_spline;
+
+int gpd_spline_create(int l, double rcut, const double* f, int npts,
+                      gpd_spline** out);
+double gpd_spline_value(const gpd_spline* s, double r);
+double gpd_spline_get_cutoff(const gpd_spline* s);
+int gpd_spline_get_angular_momentum_number(const gpd_spline* s);
+int gpd_spline_destroy(gpd_spline* s);
+
+/* spline_to_grid (c/lfc.c:785-875), host FP64.  Tabulates f(r) r^l Y_lm
+ * over the box [beg, end) of the local grid (corner gdcorner, size n) for
+ * points with r < rcut.  A_gm must hold (box points) x (2l+1) doubles and
+ * G_B 2 * (box rows) ints; *ng receives the rows written and *nB the
+ * number of G_B entries.                                               */
+int gpd_spline_to_grid(const gpd_spline* s, const long beg[3],
+                       const long end[3], const double pos_v[3],
+                       const double h_cv[9], const long n[3],
+                       const long gdcorner[3], double* A_gm, int* G_B,
+                       long* ng, long* nB);
+
+/* ------------------------------------------------------------------ */
+/* Localized functions (PAW projectors) on the grid.
+ * Replaces _gpaw.LFC(A_Wgm, M_W, G_B, W_B, dv, phase_qW) (c/lfc.c:106-186)
+ * for real (Gamma-point) functions.  Volume W has ng_W[W] rows of nm_W[W]
+ * values, concatenated row-major in A (host, FP64); (G_B, W_B) is the
+ * boundary list sorted by G (gpaw/lfc.py:276-350).  nG is the number of
+ * points of one grid and Mmax the number of functions.  The object keeps
+ * device copies of everything it needs.                                */
+typedef struct gpd_lfc gpd_lfc;
+
+int gpd_lfc_create(int nW, const double* A, const int* ng_W, const int* nm_W,
+                   const int* M_W, int nB, const int* G_B, const int* W_B,
+                   double dv, long nG, int Mmax, gpd_lfc** out);
+
+/* LFC.integrate (c/lfc.c:361-425): c_xM += dv * sum_G a_xG A_GM for nx
+ * grids; c_xM is (nx, Mmax) of the same dtype.  In FP64 the sums run in the
+ * reference's order, point by point.                                   */
+int gpd_lfc_integrate(gpd_lfc* lfc, int dtype, long nx, const void* d_a_xG,
+                      void* d_c_xM, void* stream);
+
+/* LFC.add (c/lfc.c:724-783): a_xG += sum_M A_GM c_xM, per point in the
+ * reference's volume order.                                            */
+int gpd_lfc_add(gpd_lfc* lfc, int dtype, long nx, const void* d_c_xM,
+                void* d_a_xG, void* stream);
+
+int gpd_lfc_destroy(gpd_lfc* lfc);
+
+/* ------------------------------------------------------------------ */
+/* Fused pseudo-Hamiltonian apply: out = T psi + v psi + sum_a p_a c_a for
+ * nb bands, the composite of FDWaveFunctions.apply_pseudo_hamiltonian
+ * (gpaw/wavefunctions/fd.py:77-83) and Hamiltonian.apply
+ * (gpaw/hamiltonian.py:426-460).  kin is the kinetic operator; d_v the
+ * local potential (one grid, dtype); lfc/d_c may be NULL for no projector
+ * term, else d_c holds c_xM = P_xM dH (nb, Mmax).  The per-point rounding
+ * sequence is the reference's: stencil sum, then + psi*v, then the
+ * projector terms in volume order.                                     */
+int gpd_hamiltonian_apply(gpd_operator* kin, gpd_lfc* lfc, int dtype, long nb,
+                          const void* d_psi, const void* d_v,
+                          const void* d_c, void* d_out, void* stream);
+
+/* Htpsit_nG += psit_nG * vt_G for nb grids of nG points
+ * (Hamiltonian.apply_local_potential, gpaw/hamiltonian.py:397-424): the
+ * product is rounded, then added, as the reference's numpy expression. */
+int gpd_add_local_potential(int dtype, long nb, long nG, const void* d_psi,
+                            const void* d_v, void* d_hpsi, void* stream);
+
+/* c_xM = P_xM dH for block-diagonal dH: atom a owns columns
+ * [M_a[a], M_a[a] + ni_a[a]) and the ni x ni matrix at dH[dH_off[a]]
+ * (row-major, dtype).  Sums run over i in order (gpaw/hamiltonian.py:456,
+ * there via numpy/BLAS).  d_M_a, d_ni_a, d_dH_off are device int arrays. */
+int gpd_projector_coefs(int dtype, long nb, int Mmax, int natoms,
+                        const int* d_M_a, const int* d_ni_a,
+                        const int* d_dH_off, const void* d_dH,
+                        const void* d_P, void* d_c, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Multigrid preconditioner (gpaw/preconditioner.py:11-60): a 3-level
+ * V-cycle with kin0 on the fine grid, 7-point kin1/kin2 on the coarse
+ * grids, restrictors r0 (fine->1), r1 (1->2), interpolators i2 (2->1),
+ * i1 (1->fine), step = 0.66666666 / diag(kin0).  The object owns device
+ * scratch for up to `block` bands.                                     */
+typedef struct gpd_precond gpd_precond;
+
+int gpd_precond_create(gpd_operator* kin0, gpd_operator* kin1,
+                       gpd_operator* kin2, gpd_transformer* r0,
+                       gpd_transformer* r1, gpd_transformer* i2,
+                       gpd_transformer* i1, double step, long block,
+                       gpd_precond** out);
+
+/* __call__(residuals) -> d0: out = preconditioned correction for nb <=
+ * block residuals.                                                     */
+int gpd_precond_apply(gpd_precond* pc, int dtype, long nb, const void* d_R,
+                      void* d_out, void* stream);
+
+int gpd_precond_destroy(gpd_precond* pc);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPAWDFT_AMD_H */

@@ -1,0 +1,313 @@
+"""GPU parity: every device operator against the CPU oracle on the same
+seeded inputs.  FP64 results must match bit for bit; FP32 results within
+the stated relative (2-norm) tolerance of the FP64 oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_err, same_bits, stencil_args, to_np, transformer_args
+
+pytestmark = pytest.mark.gpu
+
+Bohr = 0.52917721067
+F32_TOL = 2e-6     # relative 2-norm error of FP32 results vs the FP64 oracle
+
+
+def grid(N, pbc=True, h=0.2, cell=None):
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    N = np.array(N)
+    if cell is None:
+        cell = N * h / Bohr
+    return GridDescriptor(N, cell, pbc)
+
+
+def device(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+CASES_FD = [((48, 48, 48), True, 3), ((48, 48, 48), True, 1),
+            ((24, 20, 28), True, 2), ((20, 16, 24), (1, 0, 1), 3),
+            ((16, 20, 24), False, 6), ((12, 12, 12), True, 1),
+            ((40, 36, 44), (0, 1, 1), 4)]
+
+
+@pytest.mark.parametrize('N,pbc,nn', CASES_FD)
+def test_fd_f64_bitexact(cuda, N, pbc, nn):
+    import torch
+    from gpawdft_amd.fd_operators import Laplace
+    gd = grid(N, pbc)
+    op = Laplace(gd, -0.5, nn)
+    a = np.random.default_rng(11).standard_normal((3,) + tuple(gd.n_c))
+    out = torch.empty((3,) + tuple(gd.n_c), dtype=torch.float64, device=cuda)
+    op.apply(device(a), out)
+    ref = O.fd(*stencil_args(op)[:2], gd.pbc_c, a)
+    assert same_bits(to_np(out), ref)
+
+
+@pytest.mark.parametrize('N,pbc,nn', CASES_FD[:4])
+def test_fd_f32(cuda, N, pbc, nn):
+    import torch
+    from gpawdft_amd.fd_operators import Laplace
+    gd = grid(N, pbc)
+    op = Laplace(gd, -0.5, nn)
+    a = np.random.default_rng(12).standard_normal((2,) + tuple(gd.n_c))
+    a32 = a.astype(np.float32)
+    out = torch.empty(a.shape, dtype=torch.float32, device=cuda)
+    op.apply(device(a32), out)
+    ref = O.fd(*stencil_args(op)[:2], gd.pbc_c, a32.astype(np.float64))
+    assert rel_err(to_np(out), ref) < F32_TOL
+
+
+def test_fd_nonorthorhombic_generic(cuda):
+    """Off-axis stencil points (GUC Laplacian of a hexagonal cell) run on
+    the generic kernel with corner-filled halos."""
+    import torch
+    from gpawdft_amd.fd_operators import Laplace
+    cell = np.array([[4.0, 0, 0], [2.0, 3.4641016, 0], [0, 0, 5.0]])
+    gd = grid((20, 20, 24), True, cell=cell)
+    op = Laplace(gd, -0.5, 2)
+    assert not op.cfd
+    a = np.random.default_rng(13).standard_normal((2,) + tuple(gd.n_c))
+    out = torch.empty(a.shape, dtype=torch.float64, device=cuda)
+    op.apply(device(a), out)
+    assert same_bits(to_np(out), O.fd(op.coef_p, op.offset_pc, gd.pbc_c, a))
+
+
+def test_fd_numpy_roundtrip(cuda):
+    """Host numpy arrays are accepted (copied to HBM and back)."""
+    from gpawdft_amd.fd_operators import Laplace
+    gd = grid((24, 24, 24))
+    op = Laplace(gd, -0.5, 3)
+    a = np.random.default_rng(14).standard_normal((2,) + tuple(gd.n_c))
+    out = np.empty_like(a)
+    op.apply(a, out)
+    assert same_bits(out, O.fd(op.coef_p, op.offset_pc, gd.pbc_c, a))
+    assert op.get_diagonal_element() == op.coef_p[0]
+
+
+CASES_TR = [((48, 48, 48), True), ((20, 16, 24), (1, 0, 1)),
+            ((16, 20, 24), False), ((96, 96, 96), True)]
+
+
+@pytest.mark.parametrize('N,pbc', CASES_TR)
+@pytest.mark.parametrize('nn', [1, 2, 3, 4])
+def test_transformers_f64_bitexact(cuda, N, pbc, nn):
+    from gpawdft_amd.transformers import Transformer
+    gd = grid(N, pbc)
+    gdc = gd.coarsen()
+    nb = 1 if N[0] == 96 else 2
+    rng = np.random.default_rng(21 + nn)
+    a = rng.standard_normal((nb,) + tuple(gd.n_c))
+    r = Transformer(gd, gdc, nn)
+    out = r.apply(device(a))
+    k, pad, skip, n_in, n_out = transformer_args(r)
+    ref = O.transform(k, False, pad, skip, n_out, gd.pbc_c, a)
+    assert same_bits(to_np(out), ref), 'restrict'
+    b = rng.standard_normal((nb,) + tuple(gdc.n_c))
+    try:
+        i = Transformer(gdc, gd, nn)
+    except ValueError:
+        pytest.skip('grid too small for this interpolator')
+    out = i.apply(device(b))
+    k, pad, skip, n_in, n_out = transformer_args(i)
+    ref = O.transform(k, True, pad, skip, n_out, gd.pbc_c, b)
+    assert same_bits(to_np(out), ref), 'interpolate'
+
+
+@pytest.mark.parametrize('nn', [1, 3])
+def test_transformers_f32(cuda, nn):
+    import torch
+    from gpawdft_amd.transformers import Transformer
+    gd = grid((48, 48, 48))
+    gdc = gd.coarsen()
+    a = np.random.default_rng(31).standard_normal((2,) + tuple(gd.n_c))
+    a32 = a.astype(np.float32)
+    r = Transformer(gd, gdc, nn)
+    out = r.apply(device(a32))
+    assert out.dtype == torch.float32
+    k, pad, skip, n_in, n_out = transformer_args(r)
+    ref = O.transform(k, False, pad, skip, n_out, gd.pbc_c,
+                      a32.astype(np.float64))
+    assert rel_err(to_np(out), ref) < F32_TOL
+    i = Transformer(gdc, gd, nn)
+    b32 = ref.astype(np.float32)
+    out = i.apply(device(b32))
+    k, pad, skip, n_in, n_out = transformer_args(i)
+    ref = O.transform(k, True, pad, skip, n_out, gd.pbc_c,
+                      b32.astype(np.float64))
+    assert rel_err(to_np(out), ref) < F32_TOL
+
+
+def small_system(N=40, natoms=6, seed=5):
+    from gpawdft_amd.synthetic import System
+    return System(N, 0.18, natoms, seed)
+
+
+def test_lfc_integrate_add_f64_bitexact(cuda):
+    import torch
+    sysm = small_system()
+    kin, pt = sysm.operators()
+    gd = sysm.gd
+    rng = np.random.default_rng(41)
+    a = rng.standard_normal((3,) + tuple(gd.n_c))
+    P = torch.zeros((3, pt.Mmax), dtype=torch.float64, device=cuda)
+    pt.integrate_xM(device(a), P)
+    ref = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a,
+                          pt.Mmax)
+    assert same_bits(to_np(P), ref)
+    c = rng.standard_normal((3, pt.Mmax))
+    at = device(a)
+    pt.add_xM(device(c), at)
+    assert same_bits(to_np(at), O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B,
+                                          pt.W_B, c, a))
+
+
+def test_lfc_f32(cuda):
+    import torch
+    sysm = small_system()
+    kin, pt = sysm.operators()
+    gd = sysm.gd
+    rng = np.random.default_rng(42)
+    a = rng.standard_normal((5,) + tuple(gd.n_c)).astype(np.float32)
+    P = torch.zeros((5, pt.Mmax), dtype=torch.float32, device=cuda)
+    pt.integrate_xM(device(a), P)
+    ref = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv,
+                          a.astype(np.float64), pt.Mmax)
+    assert rel_err(to_np(P), ref) < F32_TOL
+    c = rng.standard_normal((5, pt.Mmax)).astype(np.float32)
+    at = device(a)
+    pt.add_xM(device(c), at)
+    ref = O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c.astype(np.float64),
+                    a.astype(np.float64))
+    assert rel_err(to_np(at), ref) < F32_TOL
+
+
+def test_lfc_dict_api(cuda):
+    """integrate/add through the reference's dict interface."""
+    sysm = small_system(32, 4, 9)
+    kin, pt = sysm.operators()
+    gd = sysm.gd
+    a = np.random.default_rng(43).standard_normal((2,) + tuple(gd.n_c))
+    P_ani = pt.dict(2)
+    pt.integrate(device(a), P_ani)
+    ref = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a,
+                          pt.Mmax)
+    for atom in pt.atom_indices:
+        M1, M2 = pt.M_range(atom)
+        assert same_bits(to_np(P_ani[atom]), ref[:, M1:M2])
+    b = device(a)
+    pt.add(b, P_ani)
+    assert same_bits(to_np(b), O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B,
+                                         ref, a))
+
+
+def oracle_hpsi(sysm, kin, pt, psi):
+    from gpawdft_amd.hamiltonian import unpack
+    t = O.fd(kin.coef_p, kin.offset_pc, sysm.gd.pbc_c, psi)
+    t = O.add_local_potential(psi, sysm.vt_G, t)
+    P = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, sysm.gd.dv, psi,
+                        pt.Mmax)
+    atoms = [(pt.M_a[a], unpack(sysm.dH_asp[a][0])) for a in pt.atom_indices]
+    c = O.projector_coefs(P, atoms)
+    return O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c, t), P
+
+
+def test_hamiltonian_fused_f64_bitexact(cuda):
+    import torch
+    from gpawdft_amd.hamiltonian import Hamiltonian
+    sysm = small_system(40, 6, 5)
+    kin, pt = sysm.operators()
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    from gpawdft_amd.synthetic import smooth_bands_numpy
+    psi = smooth_bands_numpy(4, sysm.gd.n_c, 3)
+    out = torch.empty(psi.shape, dtype=torch.float64, device=cuda)
+    P = ham.apply_fused(device(psi), out, kin, pt)
+    ref, Pref = oracle_hpsi(sysm, kin, pt, psi)
+    assert same_bits(to_np(P), Pref)
+    assert same_bits(to_np(out), ref)
+
+
+def test_hamiltonian_sequence_f64(cuda):
+    """Hamiltonian.apply (operator-by-operator sequence) equals the fused
+    kernel bit for bit."""
+    import torch
+    from gpawdft_amd.hamiltonian import FDWaveFunctions, Hamiltonian, KPoint
+    sysm = small_system(32, 4, 8)
+    kin, pt = sysm.operators()
+    wfs = FDWaveFunctions(sysm.gd, pt)
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    psi = device(np.random.default_rng(51).standard_normal(
+        (3,) + tuple(sysm.gd.n_c)))
+    a = torch.empty_like(psi)
+    b = torch.empty_like(psi)
+    ham.apply(psi, a, wfs, KPoint())
+    ham.apply_fused(psi, b, wfs.kin, pt)
+    assert same_bits(to_np(a), to_np(b))
+
+
+def test_hamiltonian_fused_f32(cuda):
+    import torch
+    from gpawdft_amd.hamiltonian import Hamiltonian
+    from gpawdft_amd.synthetic import smooth_bands_numpy
+    sysm = small_system(40, 6, 5)
+    kin, pt = sysm.operators()
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    psi = smooth_bands_numpy(4, sysm.gd.n_c, 3)
+    psi32 = psi.astype(np.float32)
+    out = torch.empty(psi.shape, dtype=torch.float32, device=cuda)
+    ham.apply_fused(device(psi32), out, kin, pt)
+    ref, _ = oracle_hpsi(sysm, kin, pt, psi32.astype(np.float64))
+    assert rel_err(to_np(out), ref) < 1e-6
+
+
+@pytest.mark.parametrize('N,pbc', [((48, 48, 48), True),
+                                   ((32, 40, 24), True),
+                                   ((32, 32, 40), (1, 0, 1))])
+def test_preconditioner_f64_bitexact(cuda, N, pbc):
+    import torch
+    from gpawdft_amd.fd_operators import Laplace
+    from gpawdft_amd.preconditioner import Preconditioner
+    gd = grid(N, pbc, h=0.18)
+    kin = Laplace(gd, -0.5, 3)
+    pc = Preconditioner(gd, kin, block=4)
+    R = np.random.default_rng(61).standard_normal((3,) + tuple(gd.n_c))
+    out = pc(device(R), None)
+    assert out.dtype == torch.float64
+    ref = O.precondition(stencil_args(kin), stencil_args(pc.kin1),
+                         stencil_args(pc.kin2),
+                         transformer_args(pc.restrictor_object0),
+                         transformer_args(pc.restrictor_object1),
+                         transformer_args(pc.interpolator_object2),
+                         transformer_args(pc.interpolator_object1),
+                         gd.pbc_c, pc.step, R)
+    assert same_bits(to_np(out), ref)
+
+
+def test_preconditioner_f32(cuda):
+    from gpawdft_amd.fd_operators import Laplace
+    from gpawdft_amd.preconditioner import Preconditioner
+    gd = grid((48, 48, 48), True, h=0.18)
+    kin = Laplace(gd, -0.5, 3)
+    pc = Preconditioner(gd, kin, block=2)
+    R = np.random.default_rng(62).standard_normal((2,) + tuple(gd.n_c))
+    R32 = R.astype(np.float32)
+    out = pc(device(R32), None)
+    ref = O.precondition(stencil_args(kin), stencil_args(pc.kin1),
+                         stencil_args(pc.kin2),
+                         transformer_args(pc.restrictor_object0),
+                         transformer_args(pc.restrictor_object1),
+                         transformer_args(pc.interpolator_object2),
+                         transformer_args(pc.interpolator_object1),
+                         gd.pbc_c, pc.step, R32.astype(np.float64))
+    assert rel_err(to_np(out), ref) < F32_TOL
+
+
+def test_preconditioner_block_limit(cuda):
+    from gpawdft_amd.fd_operators import Laplace
+    from gpawdft_amd.preconditioner import Preconditioner
+    gd = grid((24, 24, 24))
+    pc = Preconditioner(gd, Laplace(gd, -0.5, 3), block=2)
+    with pytest.raises(ValueError):
+        pc(device(np.zeros((3, 24, 24, 24))), None)

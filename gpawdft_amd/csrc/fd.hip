@@ -75,17 +75,36 @@ template <typename T, int R> struct AxisCoefs {
     int order[3];    // axis of each block
 };
 
+// Workgroup barrier that orders LDS only: global loads issued before it
+// stay in flight (a plain __syncthreads() would wait for them).
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Streaming axis stencil.  Per thread: two rows (ty, ty + 8) of one column
+// x of a 16 x 32 tile.  A register window of W planes per row holds the
+// axis-0 neighbours and prefetches D planes ahead; the window is rotated
+// statically (the z loop is unrolled by W), so no registers are copied.
+// The in-plane neighbours come from a double-buffered LDS tile: the centre
+// values from the window, the R-wide halo ring from loads issued two
+// planes ahead.  One LDS-only barrier per plane.
 template <typename T, int R, int EPI, bool LFC>
 __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
     const T* __restrict__ in, T* __restrict__ out, int n0, int n1, int n2,
     int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<T, R> cf,
     FdEpilogue<T> ep)
 {
+    constexpr int D = 5;                      // prefetch distance (planes)
+    constexpr int W = 2 * R + 1 + D;          // window slots (even)
+    static_assert(W % 2 == 0, "window must be even for the LDS parity");
     constexpr int LW = kTX + 2 * R;
     constexpr int LH = kTY + 2 * R;
     constexpr int NRING = 2 * R * LW + 2 * R * kTY;
-    constexpr int RING_PER_THREAD = (NRING + kThreads - 1) / kThreads;
-    __shared__ T lds[LH][LW];
+    constexpr int RPT = (NRING + kThreads - 1) / kThreads;
+    __shared__ T lds[2][LH][LW];
 
     const int tiles1 = (n1 + kTY - 1) / kTY;
     const int tiles2 = (n2 + kTX - 1) / kTX;
@@ -93,49 +112,54 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
     const long nwork = (long)nband * zchunks * tiles1 * tiles2;
     const long w = xcd_work_index(blockIdx.x, nwork);
     if (w < 0) return;
-    int t2 = (int)(w % tiles2);
+    const int t2 = (int)(w % tiles2);
     long r = w / tiles2;
-    int t1 = (int)(r % tiles1);
+    const int t1 = (int)(r % tiles1);
     r /= tiles1;
-    int zc = (int)(r % zchunks);
-    long band = r / zchunks;
+    const int zc = (int)(r % zchunks);
+    const long band = r / zchunks;
 
     const int tx = threadIdx.x % kTX;
     const int ty = threadIdx.x / kTX;          // 0..7
     const int x0 = t2 * kTX, y0 = t1 * kTY;
     const int z0 = zc * tz, z1 = min(n0, z0 + tz);
+    const int nz = z1 - z0;
     const long plane = (long)n1 * n2;
     const long nG = plane * n0;
     const T* a = in + band * nG;
     T* b = out + band * nG;
 
-    // own points: rows ty and ty + 8 of the tile
-    int gx = x0 + tx;
+    // own points: rows ty and ty + 8; out-of-grid rows of partial tiles
+    // still feed their (wrapped or zero) values to the LDS tile
+    const int gx = x0 + tx;
     bool valid[kRows];
-    long own_off[kRows];      // in-plane offset, or -1 when zero
+    long own_off[kRows];
+#pragma unroll
     for (int k = 0; k < kRows; k++) {
-        int gy = y0 + ty + 8 * k;
+        const int gy = y0 + ty + 8 * k;
         valid[k] = gx < n2 && gy < n1;
-        int wy = wrap_index(gy, n1, pbc1);
-        int wx = wrap_index(gx, n2, pbc2);
+        const int wy = wrap_index(gy, n1, pbc1);
+        const int wx = wrap_index(gx, n2, pbc2);
         own_off[k] = (wy >= 0 && wx >= 0) ? (long)wy * n2 + wx : -1;
     }
-    // ring slots of this thread
-    int ring_i[RING_PER_THREAD], ring_j[RING_PER_THREAD];
-    long ring_off[RING_PER_THREAD];
-    for (int k = 0; k < RING_PER_THREAD; k++) {
+    int ring_i[RPT], ring_j[RPT];
+    long ring_off[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
         int s = threadIdx.x + k * kThreads;
         int i = -1, j = -1;
         if (s < NRING) {
             if (s < R * LW) {                       // top rows
-                i = s / LW; j = s % LW;
+                i = s / LW;
+                j = s % LW;
             } else if (s < 2 * R * LW) {            // bottom rows
                 s -= R * LW;
-                i = kTY + R + s / LW; j = s % LW;
+                i = kTY + R + s / LW;
+                j = s % LW;
             } else {                                // side columns
                 s -= 2 * R * LW;
                 i = R + s / (2 * R);
-                int c = s % (2 * R);
+                const int c = s % (2 * R);
                 j = c < R ? c : kTX + c;
             }
         }
@@ -143,97 +167,125 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
         ring_j[k] = j;
         long o = -1;
         if (i >= 0) {
-            int wy = wrap_index(y0 - R + i, n1, pbc1);
-            int wx = wrap_index(x0 - R + j, n2, pbc2);
+            const int wy = wrap_index(y0 - R + i, n1, pbc1);
+            const int wx = wrap_index(x0 - R + j, n2, pbc2);
             if (wy >= 0 && wx >= 0) o = (long)wy * n2 + wx;
         }
         ring_off[k] = o;
     }
 
     auto load_plane = [&](int z, long off) -> T {
-        int wz = wrap_index(z, n0, pbc0);
+        const int wz = wrap_index(z, n0, pbc0);
         if (wz < 0 || off < 0) return T(0);
         return a[(long)wz * plane + off];
     };
 
-    // register window: win[k][j] = value at plane z - R + j
-    T win[kRows][2 * R + 1];
-    for (int k = 0; k < kRows; k++)
-        for (int j = 0; j < 2 * R + 1; j++)
-            win[k][j] = (valid[k] && j < 2 * R) ? load_plane(z0 - R + j, own_off[k]) : T(0);
+    // window slot s holds plane z0 - R + s (mod W)
+    T win[kRows][W];
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+#pragma unroll
+        for (int s = 0; s < W - 1; s++) win[k][s] = load_plane(z0 - R + s, own_off[k]);
+        win[k][W - 1] = T(0);
+    }
+    T ringv[2][RPT];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            ringv[q][k] = ring_i[k] >= 0 ? load_plane(z0 + q, ring_off[k]) : T(0);
 
-    for (int z = z0; z < z1; z++) {
-        // leading edge
-        for (int k = 0; k < kRows; k++)
-            if (valid[k]) win[k][2 * R] = load_plane(z + R, own_off[k]);
-        // fill LDS with plane z
-        for (int k = 0; k < kRows; k++) {
-            T v = valid[k] ? win[k][R] : load_plane(z, own_off[k]);
-            lds[R + ty + 8 * k][R + tx] = v;
-        }
-        for (int k = 0; k < RING_PER_THREAD; k++)
-            if (ring_i[k] >= 0) lds[ring_i[k]][ring_j[k]] = load_plane(z, ring_off[k]);
-        __syncthreads();
-
-        for (int k = 0; k < kRows; k++) {
-            if (!valid[k]) continue;
-            const int li = R + ty + 8 * k, lj = R + tx;
-            const T x = win[k][R];
-            T s = T(0);
-            s = madd<T>(s, x, cf.c0);
+    for (int ib = 0; ib < nz; ib += W) {
 #pragma unroll
-            for (int blk = 0; blk < 3; blk++) {
-                const int ax = cf.order[blk];
-                if (ax == 0) {
+        for (int ph = 0; ph < W; ph++) {
+            const int i = ib + ph;
+            if (i >= nz) break;
+            const int z = z0 + i;
+            const int buf = ph & 1;
+            // plane z into LDS
 #pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, win[k][R + j], cf.c[blk][j - 1]);
+            for (int k = 0; k < kRows; k++)
+                lds[buf][R + ty + 8 * k][R + tx] = win[k][(ph + R) % W];
 #pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, win[k][R - j], cf.c[blk][R + j - 1]);
-                } else if (ax == 1) {
+            for (int k = 0; k < RPT; k++)
+                if (ring_i[k] >= 0) lds[buf][ring_i[k]][ring_j[k]] = ringv[ph & 1][k];
+            lds_barrier();
+            // prefetch: window plane z + R + D, ring plane z + 2
+            if (i + D < nz) {
 #pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li + j][lj], cf.c[blk][j - 1]);
-#pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li - j][lj], cf.c[blk][R + j - 1]);
-                } else {
-#pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li][lj + j], cf.c[blk][j - 1]);
-#pragma unroll
-                    for (int j = 1; j <= R; j++) s = madd<T>(s, lds[li][lj - j], cf.c[blk][R + j - 1]);
-                }
+                for (int k = 0; k < kRows; k++)
+                    win[k][(ph + W - 1) % W] = load_plane(z + R + D, own_off[k]);
             }
-            const long p = (long)z * plane + own_off[k];
-            T o;
-            if (EPI == EPI_PLAIN) {
-                o = s;
-            } else if (EPI == EPI_POT) {
-                o = madd<T>(s, x, ep.aux[p]);
-                if (LFC) {
-                    const int e0 = ep.lfc_ptr[p], e1 = ep.lfc_ptr[p + 1];
-                    const T* cb = ep.lfc_c + band * ep.Mmax;
-                    for (int e = e0; e < e1; e++) {
-                        const int2 en = ep.lfc_ent[e];
-                        const int nm = en.y & 15;
-                        const T* Ar = ep.lfc_A + en.x;
-                        const T* cm = cb + (en.y >> 4);
-                        for (int m = 0; m < nm; m++) o = madd<T>(o, Ar[m], cm[m]);
+            if (i + 2 < nz) {
+#pragma unroll
+                for (int k = 0; k < RPT; k++)
+                    if (ring_i[k] >= 0) ringv[ph & 1][k] = load_plane(z + 2, ring_off[k]);
+            }
+            // compute
+#pragma unroll
+            for (int k = 0; k < kRows; k++) {
+                if (!valid[k]) continue;
+                const int li = R + ty + 8 * k, lj = R + tx;
+                const T x = win[k][(ph + R) % W];
+                T s = T(0);
+                s = madd<T>(s, x, cf.c0);
+#pragma unroll
+                for (int blk = 0; blk < 3; blk++) {
+                    const int ax = cf.order[blk];
+                    if (ax == 0) {
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, win[k][(ph + R + j) % W], cf.c[blk][j - 1]);
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, win[k][(ph + R - j) % W], cf.c[blk][R + j - 1]);
+                    } else if (ax == 1) {
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, lds[buf][li + j][lj], cf.c[blk][j - 1]);
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, lds[buf][li - j][lj], cf.c[blk][R + j - 1]);
+                    } else {
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, lds[buf][li][lj + j], cf.c[blk][j - 1]);
+#pragma unroll
+                        for (int j = 1; j <= R; j++)
+                            s = madd<T>(s, lds[buf][li][lj - j], cf.c[blk][R + j - 1]);
                     }
                 }
-            } else if (EPI == EPI_SUB) {
-                o = s - ep.aux[band * nG + p];
-            } else {
-                T q = s - ep.aux[band * nG + p];
-                o = madd<T>(x, ep.alpha, q);
-                if (ep.neg) o = -o;
+                const long p = (long)z * plane + own_off[k];
+                T o;
+                if (EPI == EPI_PLAIN) {
+                    o = s;
+                } else if (EPI == EPI_POT) {
+                    o = madd<T>(s, x, ep.aux[p]);
+                    if (LFC) {
+                        const int e0 = ep.lfc_ptr[p], e1 = ep.lfc_ptr[p + 1];
+                        const T* cb = ep.lfc_c + band * ep.Mmax;
+                        for (int e = e0; e < e1; e++) {
+                            const int2 en = ep.lfc_ent[e];
+                            const int nm = en.y & 15;
+                            const T* Ar = ep.lfc_A + en.x;
+                            const T* cm = cb + (en.y >> 4);
+                            for (int m = 0; m < nm; m++) o = madd<T>(o, Ar[m], cm[m]);
+                        }
+                    }
+                } else if (EPI == EPI_SUB) {
+                    o = s - ep.aux[band * nG + p];
+                } else {
+                    const T q = s - ep.aux[band * nG + p];
+                    o = madd<T>(x, ep.alpha, q);
+                    if (ep.neg) o = -o;
+                }
+                b[p] = o;
             }
-            b[p] = o;
         }
-        __syncthreads();
-        for (int k = 0; k < kRows; k++)
-#pragma unroll
-            for (int j = 0; j < 2 * R; j++) win[k][j] = win[k][j + 1];
     }
 }
 
+// ------------------------------------------------------------------ //
 // Epilogue for stencils without the axis form: s was computed by the
 // generic kernel into `s_buf`; same per-point arithmetic as the axis kernel.
 template <typename T, int EPI>
@@ -285,11 +337,14 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
 
 static int pick_tz(int n0, long work_per_plane_chunk, int R)
 {
-    // Enough workgroups to fill 256 CUs several times over, but chunks at
-    // least 4R planes deep so the warm-up of the register window stays cheap.
-    long target = 4096;
-    int nz = (int)std::max<long>(1, std::min<long>(n0 / std::max(4 * R, 8), target / std::max<long>(1, work_per_plane_chunk)));
-    return (n0 + nz - 1) / nz;
+    // Split axis 0 into chunks until there are ~8192 workgroups (about four
+    // rounds of the 2048 that fit on the chip), keeping chunks at least
+    // max(16, 4R) planes deep so the window warm-up stays cheap.
+    const long target = 8192;
+    const int min_depth = std::max(16, 4 * R);
+    long nz = (target + work_per_plane_chunk - 1) / std::max<long>(1, work_per_plane_chunk);
+    nz = std::max<long>(1, std::min<long>(nz, n0 / min_depth));
+    return (int)((n0 + nz - 1) / nz);
 }
 
 template <typename T, int R, int EPI, bool LFC>

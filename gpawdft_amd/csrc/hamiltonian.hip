@@ -7,6 +7,8 @@
 // epilogue, so psi is read once and H psi written once; the per-point
 // rounding sequence is the reference's (stencil sum, + psi*v, then the
 // projector terms in volume order).
+#include <type_traits>
+
 #include "common.hpp"
 #include "fd.hpp"
 #include "lfc.hpp"
@@ -44,12 +46,20 @@ __global__ void local_potential_kernel(long nb, long nG, const T* __restrict__ p
     hpsi[idx] = madd<T>(hpsi[idx], psi[idx], v[idx % nG]);
 }
 
+// FP64: one launch, projector terms in the stencil epilogue in the
+// reference's per-point order (bit-compatible).  FP32: stencil + v psi,
+// then the projector terms by colours of disjoint atom groups.
 template <typename T>
 static int ham_apply(gpd_operator* kin, gpd_lfc* lfc, long nb, const T* psi, const T* v,
                      const T* c, T* out, hipStream_t st)
 {
     FdEpilogue<T> ep = {};
     ep.aux = v;
+    if (std::is_same<T, float>::value && lfc && c) {
+        int rc = fd_apply_typed<T>(kin, EPI_POT, nb, psi, out, ep, st);
+        if (rc) return rc;
+        return lfc_add_grouped_f32(lfc, nb, (const float*)c, (float*)out, st);
+    }
     if (lfc && c) {
         ep.lfc_ptr = lfc->ptr;
         ep.lfc_ent = lfc->ent;

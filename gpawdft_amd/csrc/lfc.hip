@@ -34,7 +34,10 @@ constexpr int kNF = 16;   // functions per group held in registers
 // ------------------------------------------------------------------ //
 // kernels
 
-// FP64-exact integrate: one thread per (band, function block).
+// FP64-exact integrate: one thread per (band, function block), bands
+// fastest so that a wave shares the block's point list (scalar loads).
+// Points are taken in batches of 8 with all loads issued first; the adds
+// into c stay in the reference's sequence.
 template <typename T>
 __global__ void lfc_integrate_seq_kernel(const T* __restrict__ a, T* __restrict__ c, long nG,
                                          int Mmax, int nkeys, long nx, T dv,
@@ -45,23 +48,42 @@ __global__ void lfc_integrate_seq_kernel(const T* __restrict__ a, T* __restrict_
                                          const int* __restrict__ key_G,
                                          const int* __restrict__ key_row)
 {
+    constexpr int BATCH = 8;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nx * nkeys) return;
-    const int k = (int)(idx % nkeys);
-    const long band = idx / nkeys;
+    const long band = idx % nx;
+    const int k = (int)(idx / nx);
     const int M = key_M[k], nm = key_nm[k];
     const T* ab = a + band * nG;
     T* cb = c + band * Mmax + M;
     T acc[kNF];
 #pragma unroll
     for (int m = 0; m < kNF; m++) acc[m] = m < nm ? cb[m] : T(0);
-    const long j1 = key_off[k + 1];
-    for (long j = key_off[k]; j < j1; j++) {
-        const T av = ab[key_G[j]] * dv;
+    const long j0 = key_off[k], j1 = key_off[k + 1];
+    long j = j0;
+    for (; j + BATCH <= j1; j += BATCH) {
+        T av[BATCH];
+        int row[BATCH];
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) {
+            av[q] = ab[key_G[j + q]];
+            row[q] = key_row[j + q];
+        }
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) {
+            const T x = av[q] * dv;
+            const T* Ar = A + row[q];
+#pragma unroll
+            for (int m = 0; m < kNF; m++)
+                if (m < nm) acc[m] = madd<T>(acc[m], x, Ar[m]);
+        }
+    }
+    for (; j < j1; j++) {
+        const T x = ab[key_G[j]] * dv;
         const T* Ar = A + key_row[j];
 #pragma unroll
         for (int m = 0; m < kNF; m++)
-            if (m < nm) acc[m] = madd<T>(acc[m], av, Ar[m]);
+            if (m < nm) acc[m] = madd<T>(acc[m], x, Ar[m]);
     }
 #pragma unroll
     for (int m = 0; m < kNF; m++)
@@ -125,6 +147,47 @@ __global__ __launch_bounds__(256) void lfc_integrate_grp_kernel(
             T* cp = c + band * Mmax + grp_fM[g * nfmax + f];
             *cp = *cp + (T)s * dv;
         }
+    }
+}
+
+// Grouped add (FP32): one thread per point of a group, NB bands per
+// workgroup; all functions of the group are summed, then added once.
+// Groups of one launch ("colour") cover disjoint points, so the
+// read-modify-writes do not race.
+template <int NB>
+__global__ __launch_bounds__(256) void lfc_add_grp_kernel(
+    const float* __restrict__ c, float* __restrict__ a, long nG, int Mmax, long nx, int nfmax,
+    int nitems, const int* __restrict__ item_g, const long* __restrict__ item_j0,
+    const long* __restrict__ grp_off, const int* __restrict__ grp_nf,
+    const int* __restrict__ grp_fM, const int* __restrict__ grp_G,
+    const float* __restrict__ grp_A)
+{
+    const long nbblk = (nx + NB - 1) / NB;
+    const long w = xcd_work_index(blockIdx.x, (long)nitems * nbblk);
+    if (w < 0) return;
+    const int item = (int)(w % nitems);
+    const long b0 = (w / nitems) * NB;
+    const int g = item_g[item];
+    const long j = item_j0[item] + threadIdx.x;
+    if (j >= grp_off[g + 1]) return;
+    const int nf = grp_nf[g];
+    const long G = grp_G[j];
+    const float* Ar = grp_A + j * nfmax;
+    const int* fM = grp_fM + (long)g * nfmax;
+    float av[kNF];
+#pragma unroll
+    for (int f = 0; f < kNF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+#pragma unroll
+    for (int bb = 0; bb < NB; bb++) {
+        const long b = b0 + bb;
+        if (b >= nx) break;
+        const float* cb = c + b * Mmax;
+        float s = 0.f;
+#pragma unroll
+        for (int f = 0; f < kNF; f++)
+            if (f < nf) s = __builtin_fmaf(av[f], cb[fM[f]], s);
+        float* ap = a + b * nG + G;
+        *ap = *ap + s;
     }
 }
 
@@ -276,6 +339,24 @@ static hipError_t upload(V** dst, const V* src, size_t n)
     return hipMemset(*dst, 0, sizeof(V) * n);
 }
 
+int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st)
+{
+    constexpr int NB = 8;
+    const long nbblk = (nx + NB - 1) / NB;
+    for (int col = 0; col < l->ncolours; col++) {
+        const int i0 = l->colour_items[col], i1 = l->colour_items[col + 1];
+        if (i1 == i0) continue;
+        const long nwork = (long)(i1 - i0) * nbblk;
+        hipLaunchKernelGGL(lfc_add_grp_kernel<NB>, dim3(xcd_grid_size(nwork)), dim3(256), 0, st, c,
+                           a, l->nG, l->Mmax, nx, l->nfmax, i1 - i0, l->item_g + i0,
+                           l->item_j0 + i0, l->grp_off, l->grp_nf, l->grp_fM, l->grp_G,
+                           l->grp_A32);
+        int rc = last_error_to_status(hipGetLastError());
+        if (rc) return rc;
+    }
+    return GPD_OK;
+}
+
 }  // namespace gpd
 
 using namespace gpd;
@@ -285,7 +366,7 @@ extern "C" int gpd_lfc_destroy(gpd_lfc* l)
     if (!l) return GPD_OK;
     void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
                     l->key_off, l->key_G, l->key_row, l->grp_off, l->grp_nf, l->grp_fM,
-                    l->grp_G, l->grp_A32};
+                    l->grp_G, l->grp_A32, l->item_g, l->item_j0};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete l;
@@ -380,6 +461,40 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     }
     std::vector<float> A32(A, A + l->nvalues);
 
+    // colour the groups: groups of one colour cover disjoint points
+    std::vector<int> colour_of(l->ngroups, -1);
+    std::vector<std::vector<char>> used;
+    for (int g = 0; g < l->ngroups; g++) {
+        const std::vector<int>& pts = h.key_G[groups[g][0]];
+        int col = 0;
+        for (;; col++) {
+            if (col == (int)used.size()) used.emplace_back(nG, 0);
+            bool clash = false;
+            for (int G : pts)
+                if (used[col][G]) {
+                    clash = true;
+                    break;
+                }
+            if (!clash) break;
+        }
+        for (int G : pts) used[col][G] = 1;
+        colour_of[g] = col;
+    }
+    l->ncolours = (int)used.size();
+    std::vector<int> item_g;
+    std::vector<long> item_j0;
+    l->colour_items.assign(l->ncolours + 1, 0);
+    for (int col = 0; col < l->ncolours; col++) {
+        l->colour_items[col] = (int)item_g.size();
+        for (int g = 0; g < l->ngroups; g++)
+            if (colour_of[g] == col)
+                for (long j = grp_off[g]; j < grp_off[g + 1]; j += 256) {
+                    item_g.push_back(g);
+                    item_j0.push_back(j);
+                }
+    }
+    l->colour_items[l->ncolours] = (int)item_g.size();
+
     hipError_t e = hipSuccess;
 #define UP(dst, src, n) \
     if (e == hipSuccess) e = upload(&l->dst, src, n)
@@ -399,6 +514,8 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     UP(grp_fM, gfM.data(), gfM.size());
     UP(grp_G, gG.data(), gG.size());
     UP(grp_A32, gA.data(), gA.size());
+    UP(item_g, item_g.data(), item_g.size());
+    UP(item_j0, item_j0.data(), item_j0.size());
 #undef UP
     if (e != hipSuccess) {
         set_last_error(hipGetErrorString(e));
@@ -451,11 +568,7 @@ extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, v
                            l->A64, l->cov_G, l->cov_ptr, l->ent);
         return last_error_to_status(hipGetLastError());
     }
-    if (dtype == GPD_F32) {
-        hipLaunchKernelGGL(lfc_add_kernel<float>, dim3((n + 255) / 256), dim3(256), 0, st,
-                           (const float*)d_c_xM, (float*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
-                           l->A32, l->cov_G, l->cov_ptr, l->ent);
-        return last_error_to_status(hipGetLastError());
-    }
+    if (dtype == GPD_F32)
+        return lfc_add_grouped_f32(l, nx, (const float*)d_c_xM, (float*)d_a_xG, st);
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
 }

@@ -1,6 +1,8 @@
 // Internal interface of the localized-function (projector) kernels.
 #pragma once
 
+#include <vector>
+
 #include "common.hpp"
 
 // Device layout built once per geometry from the reference's boundary list.
@@ -44,8 +46,16 @@ struct gpd_lfc {
     int* grp_fM = nullptr;     // ngroups * nfmax: M index of each function
     int* grp_G = nullptr;      // npts_total
     float* grp_A32 = nullptr;  // npts_total * nfmax (row-major, zero padded)
+    // grouped add: colours of groups with disjoint points; work items are
+    // (group, first point) chunks of 256 points, listed colour by colour
+    int ncolours = 0;
+    std::vector<int> colour_items;   // host: ncolours + 1 offsets into items
+    int* item_g = nullptr;
+    long* item_j0 = nullptr;
 };
 
 namespace gpd {
 template <typename T> const T* lfc_values(const gpd_lfc* l);
+// FP32 add by colours of disjoint groups (fast path, order not preserved)
+int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st);
 }

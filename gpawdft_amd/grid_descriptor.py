@@ -14,7 +14,7 @@ import numbers
 
 import numpy as np
 
-from gpawdft_amd import device
+from gpawdft_amd import device as _dev
 
 
 class GridBoundsError(ValueError):
@@ -151,12 +151,12 @@ class GridDescriptor:
     def zeros(self, n=(), dtype=float, global_array=False, pad=False,
               device=None):
         assert not global_array and not pad
-        return device_zeros(self._shape(n), dtype, device)
+        return _dev.zeros(self._shape(n), dtype, device)
 
     def empty(self, n=(), dtype=float, global_array=False, pad=False,
               device=None):
         assert not global_array and not pad
-        return device.empty(self._shape(n), dtype, device)
+        return _dev.empty(self._shape(n), dtype, device)
 
     def coarsen(self):
         if (self.N_c % 2).any():
@@ -240,6 +240,3 @@ class GridDescriptor:
                         boxes.append((beg, end, disp))
         return boxes
 
-
-def device_zeros(shape, dtype=float, dev=None):
-    return device.zeros(shape, dtype, dev)

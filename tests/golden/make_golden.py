@@ -1,0 +1,237 @@
+"""Write the golden fixtures of this path from the reference's own C code.
+
+Runs only in the build container, where /root/reference exists: it builds
+the reference's serial C subset (oracle/ref/Makefile -> oracle/_ref) and
+drives its Operator, Transformer, Spline, spline_to_grid and LFC objects on
+seeded inputs.  The argument preparation that the reference does in Python
+(Laplace coefficients, transformer paddings, LFC boundary lists) uses
+gpawdft_amd's host set-up, which mirrors it; the reference Python cannot be
+imported here (ase is absent, gpaw/hs_operators.py does not parse on
+Python 3).
+
+Stored: seeds, shapes, the coefficient bits used, and SHA-256 digests of
+the FP64 output bytes (full arrays for the small ones).  Inputs are
+regenerated from the seeds by the tests (numpy PCG64 streams are stable).
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+
+import reference_build  # noqa: E402
+from gpawdft_amd.fd_operators import guc_laplace_stencil  # noqa: E402
+from gpawdft_amd.grid_descriptor import GridDescriptor  # noqa: E402
+from gpawdft_amd.transformers import transformer_paddings  # noqa: E402
+
+Bohr = 0.52917721067
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()
+                          ).hexdigest()
+
+
+def hexbits(a):
+    return [float(x).hex() for x in np.asarray(a, np.float64).ravel()]
+
+
+def grid(N, pbc, h=0.2, cell=None):
+    N = np.array(N)
+    return GridDescriptor(N, N * h / Bohr if cell is None else cell, pbc)
+
+
+def ref_operator(R, gd, coefs, offs):
+    offs = np.asarray(offs, int).reshape(-1, 3)
+    cfd = not any((o != 0).sum() >= 2 for o in offs)
+    mp = int(offs.max())
+    M = gd.n_c + 2 * mp
+    stride = np.array([M[1] * M[2], M[2], 1])
+    return R.Operator(np.ascontiguousarray(coefs, float),
+                      np.ascontiguousarray(offs @ stride, np.int64),
+                      np.ascontiguousarray(gd.n_c, np.int64), mp,
+                      np.ascontiguousarray(gd.neighbor_cd, np.int64), 1, None,
+                      int(cfd))
+
+
+def ref_transformer(R, gdin, gdout, nn):
+    pad, npad, skip, interp = transformer_paddings(gdin, gdout, nn)
+    i64 = lambda x: np.ascontiguousarray(x, np.int64)
+    t = R.Transformer(i64(gdin.n_c), i64(gdout.n_c), 2 * nn, i64(pad),
+                      i64(npad), i64(skip), i64(gdin.neighbor_cd), 1, None,
+                      int(interp))
+    return t
+
+
+def apply(obj, a, out_shape):
+    out = np.empty(a.shape[:-3] + tuple(int(x) for x in out_shape))
+    obj.apply(a, out)
+    return out
+
+
+def main():
+    if not reference_build.available():
+        sys.exit('the reference tree is not present; fixtures unchanged')
+    reference_build.build()
+    R = reference_build.load()
+    fx = {'bohr': Bohr, 'generator': 'numpy.random.default_rng(seed)'
+                                     '.standard_normal(shape)'}
+
+    # C1: 48^3 periodic, h = 0.2 A, 16 bands; Laplace(-0.5, nn=3),
+    # restrict 48->24 and interpolate 24->48 (nn=3)
+    gd = grid((48, 48, 48), True)
+    gdc = gd.coarsen()
+    seed = 20240
+    psi = np.random.default_rng(seed).standard_normal((16, 48, 48, 48))
+    coefs, offs, _ = guc_laplace_stencil(gd, -0.5, 3)
+    t = apply(ref_operator(R, gd, coefs, offs), psi, gd.n_c)
+    r = apply(ref_transformer(R, gd, gdc, 3), psi, gdc.n_c)
+    i = apply(ref_transformer(R, gdc, gd, 3), r, gd.n_c)
+    fx['c1'] = {'N': 48, 'h_ang': 0.2, 'bands': 16, 'seed': seed, 'nn': 3,
+                'coefs': hexbits(coefs),
+                'offsets': np.asarray(offs, int).tolist(),
+                'kinetic_sha256': digest(t), 'restrict_sha256': digest(r),
+                'interpolate_sha256': digest(i),
+                'kinetic_sample': hexbits(t[3, 7, 11, :8]),
+                'restrict_sample': hexbits(r[5, 2, 3, :8]),
+                'interpolate_sample': hexbits(i[9, 40, 1, :8])}
+
+    # small grids: mixed / non-periodic boundaries, all stencil ranges and
+    # transformer orders
+    cases = []
+    for N, pbc in (((20, 16, 24), (1, 0, 1)), ((16, 20, 24), (0, 0, 0)),
+                   ((24, 20, 28), (1, 1, 1))):
+        gd = grid(N, pbc)
+        gdc = gd.coarsen()
+        seed = 100 + len(cases)
+        a = np.random.default_rng(seed).standard_normal((2,) + tuple(gd.n_c))
+        case = {'N': list(N), 'pbc': list(pbc), 'seed': seed, 'fd': {},
+                'restrict': {}, 'interpolate': {}}
+        for nn in (1, 2, 3, 6):
+            c, o, _ = guc_laplace_stencil(gd, -0.5, nn)
+            case['fd'][str(nn)] = digest(apply(ref_operator(R, gd, c, o), a,
+                                               gd.n_c))
+        for nn in (1, 2, 3, 4):
+            rr = apply(ref_transformer(R, gd, gdc, nn), a, gdc.n_c)
+            case['restrict'][str(nn)] = digest(rr)
+            try:
+                ii = apply(ref_transformer(R, gdc, gd, nn), rr, gd.n_c)
+            except ValueError:
+                continue
+            case['interpolate'][str(nn)] = digest(ii)
+        cases.append(case)
+    fx['grids'] = cases
+
+    # non-orthorhombic cell: off-axis stencil points, corner halos
+    cell = np.array([[4.0, 0, 0], [2.0, 3.4641016, 0], [0, 0, 5.0]])
+    gd = GridDescriptor((20, 20, 24), cell, True)
+    a = np.random.default_rng(300).standard_normal((2,) + tuple(gd.n_c))
+    c, o, D = guc_laplace_stencil(gd, -0.5, 2)
+    fx['hexagonal'] = {'cell': cell.tolist(), 'N': [20, 20, 24], 'seed': 300,
+                       'nn': 2, 'directions': D, 'coefs': hexbits(c),
+                       'offsets': np.asarray(o, int).tolist(),
+                       'fd_sha256': digest(apply(ref_operator(R, gd, c, o),
+                                                 a, gd.n_c))}
+
+    # splines and spline_to_grid
+    rc, rmax = 1.2 / Bohr, 2.5 / Bohr
+    rg = np.linspace(0, rmax, 201)
+    f = np.exp(-rg**2 / rc**2)
+    f[-1] = 0.0
+    rs = np.random.default_rng(400).uniform(0, 1.05 * rmax, 64)
+    fx['spline'] = {'rc': rc, 'rmax': rmax, 'npts': 201, 'r': hexbits(rs),
+                    'values': {str(l): hexbits([R.Spline(l, rmax, f)(x)
+                                                for x in rs])
+                               for l in range(5)}}
+    gd = grid((40, 40, 40), True, h=0.18)
+    pos_v = np.array([3.1, 4.7, 2.2])
+    beg = np.array([-5, 0, 3], np.int64)
+    end = np.array([12, 17, 20], np.int64)
+    s2g = {}
+    for l in range(5):
+        A, G = R.spline_to_grid(R.Spline(l, rmax, f), beg, end, pos_v,
+                                np.ascontiguousarray(gd.h_cv),
+                                np.ascontiguousarray(gd.n_c, np.int64),
+                                np.ascontiguousarray(gd.beg_c, np.int64))
+        s2g[str(l)] = {'A_sha256': digest(A), 'rows': int(A.shape[0]),
+                       'G_B': G.tolist()}
+    fx['spline_to_grid'] = {'N': 40, 'h_ang': 0.18, 'pos_v': pos_v.tolist(),
+                            'beg': beg.tolist(), 'end': end.tolist(),
+                            'by_l': s2g}
+
+    # LFC: 40^3, 6 atoms with l = 0..2 Gaussian projectors
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import projector_splines
+    splines = projector_splines(2)
+    spos = np.random.default_rng(500).uniform(0, 1, (6, 3))
+    pt = LFC(gd, [splines] * 6)
+    pt.set_positions(spos)
+    lfc = R.LFC(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv,
+                np.empty((0, len(pt.A_Wgm)), complex))
+    rng = np.random.default_rng(501)
+    a = rng.standard_normal((3, 40, 40, 40))
+    c_xM = np.zeros((3, pt.Mmax))
+    lfc.integrate(a, c_xM, -1)
+    cc = rng.standard_normal((3, pt.Mmax))
+    b = a.copy()
+    lfc.add(cc, b, -1)
+    fx['lfc'] = {'N': 40, 'h_ang': 0.18, 'spos_seed': 500, 'natoms': 6,
+                 'data_seed': 501, 'Mmax': int(pt.Mmax),
+                 'nW': len(pt.A_Wgm), 'G_B_sha256': hashlib.sha256(
+                     pt.G_B.tobytes()).hexdigest(),
+                 'integrate': hexbits(c_xM), 'add_sha256': digest(b)}
+
+    # multigrid preconditioner sequence (gpaw/preconditioner.py:34-60) on
+    # the reference's C operators and transformers; the elementwise steps
+    # are numpy operations (two roundings; the reference's final axpy goes
+    # through BLAS daxpy)
+    gd0 = grid((48, 48, 48), True, h=0.18)
+    gd1 = gd0.coarsen()
+    gd2 = gd1.coarsen()
+    c0, o0, _ = guc_laplace_stencil(gd0, -0.5, 3)
+    c1, o1, _ = guc_laplace_stencil(gd1, -0.5, 1)
+    c2, o2, _ = guc_laplace_stencil(gd2, -0.5, 1)
+    kin0 = ref_operator(R, gd0, c0, o0)
+    kin1 = ref_operator(R, gd1, c1, o1)
+    kin2 = ref_operator(R, gd2, c2, o2)
+    step = 0.66666666 / c0[0]
+    Rr = np.random.default_rng(600).standard_normal((3, 48, 48, 48))
+    r1 = apply(ref_transformer(R, gd0, gd1, 1), -Rr, gd1.n_c)
+    d1 = 4 * step * r1
+    q1 = apply(kin1, d1, gd1.n_c)
+    q1 -= r1
+    r2 = apply(ref_transformer(R, gd1, gd2, 1), q1, gd2.n_c)
+    d2 = 16 * step * r2
+    q2 = apply(kin2, d2, gd2.n_c)
+    q2 -= r2
+    d2 -= 16 * step * q2
+    q1 = apply(ref_transformer(R, gd2, gd1, 1), d2, gd1.n_c)
+    d1 -= q1
+    q1 = apply(kin1, d1, gd1.n_c)
+    q1 -= r1
+    d1 -= 4 * step * q1
+    d0 = apply(ref_transformer(R, gd1, gd0, 1), -d1, gd0.n_c)
+    q0 = apply(kin0, d0, gd0.n_c)
+    q0 -= Rr
+    d0 += (-step) * q0
+    d0 *= -1.0
+    fx['preconditioner'] = {'N': 48, 'h_ang': 0.18, 'seed': 600, 'bands': 3,
+                            'step': float(step).hex(),
+                            'out_sha256': digest(d0),
+                            'sample': hexbits(d0[1, 5, 6, :8])}
+
+    with open(os.path.join(HERE, 'reference_fixtures.json'), 'w') as fh:
+        json.dump(fx, fh, indent=1)
+    print('wrote', os.path.join(HERE, 'reference_fixtures.json'))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,58 @@
+"""The C ABI library loads without a GPU and exports every function that
+include/gpawdft_amd.h declares (no compute calls here)."""
+import os
+import re
+
+import gpawdft_amd
+from gpawdft_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, 'include', 'gpawdft_amd.h')).read()
+    src = re.sub(r'/\*.*?\*/', '', src, flags=re.S)
+    return sorted(set(re.findall(r'\b(gpd_[a-z0-9_]+)\s*\(', src)))
+
+
+def test_header_symbols_exported():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    # the ctypes table covers the same set
+    assert set(_lib.exported_symbols()) == set(names)
+
+
+def test_version_and_error_string():
+    L = _lib.lib()
+    assert b'gfx950' in L.gpd_version()
+    assert isinstance(L.gpd_last_error(), bytes)
+    assert gpawdft_amd.library_loaded()
+
+
+def test_host_spline_matches_oracle():
+    """gpd_spline_* and gpd_spline_to_grid are host FP64 code: compare
+    them with the oracle bit for bit."""
+    import numpy as np
+    import oracle as O
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.lfc import spline_to_grid
+    from gpawdft_amd.spline import Spline
+    rmax = 4.7
+    r = np.linspace(0, rmax, 151)
+    f = np.exp(-r**2 / 5.1) * (1 + 0.3 * r)
+    for l in range(5):
+        s, o = Spline(l, rmax, f), O.Spline(l, rmax, f)
+        assert s.get_cutoff() == o.get_cutoff()
+        assert s.get_angular_momentum_number() == l
+        xs = np.random.default_rng(l).uniform(0, 1.1 * rmax, 50)
+        assert all(s(x) == o(x) for x in xs)
+        gd = GridDescriptor((30, 28, 32), (9.0, 8.5, 9.6), (1, 0, 1))
+        beg, end = np.array([-3, 1, 20]), np.array([14, 19, 39])
+        spos = np.array([0.2, 0.35, 0.9])
+        A, G = spline_to_grid(s, gd, beg, end, spos)
+        Ao, Go = o.to_grid(beg, end, np.dot(spos, gd.cell_cv), gd.h_cv,
+                           gd.n_c, gd.beg_c)
+        assert np.array_equal(A, Ao) and np.array_equal(G, Go)

@@ -1,0 +1,151 @@
+"""Pin the CPU oracle to the reference's own outputs (golden fixtures
+written by tests/golden/make_golden.py from the reference's compiled C).
+Every comparison is bit for bit (SHA-256 of the FP64 bytes)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gpawdft_amd.fd_operators import guc_laplace_stencil
+from gpawdft_amd.grid_descriptor import GridDescriptor
+from gpawdft_amd.transformers import transformer_paddings
+
+FX = json.load(open(os.path.join(os.path.dirname(__file__), 'golden',
+                                 'reference_fixtures.json')))
+Bohr = FX['bohr']
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()
+                          ).hexdigest()
+
+
+def unhex(v):
+    return np.array([float.fromhex(x) for x in v])
+
+
+def grid(N, pbc, h=0.2):
+    N = np.array(N)
+    return GridDescriptor(N, N * h / Bohr, pbc)
+
+
+def rnd(seed, shape):
+    return np.random.default_rng(seed).standard_normal(shape)
+
+
+def transform(gdin, gdout, nn, a):
+    pad, npad, skip, interp = transformer_paddings(gdin, gdout, nn)
+    return O.transform(2 * nn, interp, pad, skip, gdout.n_c, gdin.pbc_c, a)
+
+
+def test_c1_kinetic_restrict_interpolate():
+    c1 = FX['c1']
+    gd = grid((48, 48, 48), True, c1['h_ang'])
+    gdc = gd.coarsen()
+    coefs, offs, _ = guc_laplace_stencil(gd, -0.5, c1['nn'])
+    # the coefficients themselves are the reference's bits
+    assert np.array_equal(np.array(coefs), unhex(c1['coefs']))
+    assert np.asarray(offs).tolist() == c1['offsets']
+    psi = rnd(c1['seed'], (16, 48, 48, 48))
+    t = O.fd(coefs, offs, gd.pbc_c, psi)
+    assert digest(t) == c1['kinetic_sha256']
+    r = transform(gd, gdc, 3, psi)
+    assert digest(r) == c1['restrict_sha256']
+    i = transform(gdc, gd, 3, r)
+    assert digest(i) == c1['interpolate_sha256']
+
+
+@pytest.mark.parametrize('k', range(3))
+def test_small_grids(k):
+    case = FX['grids'][k]
+    gd = grid(case['N'], tuple(case['pbc']))
+    gdc = gd.coarsen()
+    a = rnd(case['seed'], (2,) + tuple(gd.n_c))
+    for nn, h in case['fd'].items():
+        c, o, _ = guc_laplace_stencil(gd, -0.5, int(nn))
+        assert digest(O.fd(c, o, gd.pbc_c, a)) == h, 'fd nn=%s' % nn
+    for nn, h in case['restrict'].items():
+        r = transform(gd, gdc, int(nn), a)
+        assert digest(r) == h, 'restrict nn=%s' % nn
+        if nn in case['interpolate']:
+            i = transform(gdc, gd, int(nn), r)
+            assert digest(i) == case['interpolate'][nn], 'interp nn=%s' % nn
+
+
+def test_hexagonal_cell_stencil():
+    hx = FX['hexagonal']
+    gd = GridDescriptor(hx['N'], np.array(hx['cell']), True)
+    c, o, D = guc_laplace_stencil(gd, -0.5, hx['nn'])
+    assert D == hx['directions']
+    assert np.array_equal(np.array(c), unhex(hx['coefs']))
+    a = rnd(hx['seed'], (2,) + tuple(gd.n_c))
+    assert digest(O.fd(c, o, gd.pbc_c, a)) == hx['fd_sha256']
+
+
+def test_splines():
+    sp = FX['spline']
+    rg = np.linspace(0, sp['rmax'], sp['npts'])
+    f = np.exp(-rg**2 / sp['rc']**2)
+    rs = unhex(sp['r'])
+    for l, vals in sp['values'].items():
+        s = O.Spline(int(l), sp['rmax'], f)
+        got = np.array([s(x) for x in rs])
+        assert np.array_equal(got, unhex(vals)), 'l=%s' % l
+
+
+def test_spline_to_grid():
+    st = FX['spline_to_grid']
+    sp = FX['spline']
+    gd = grid((st['N'],) * 3, True, st['h_ang'])
+    rg = np.linspace(0, sp['rmax'], sp['npts'])
+    f = np.exp(-rg**2 / sp['rc']**2)
+    for l, ref in st['by_l'].items():
+        A, G = O.Spline(int(l), sp['rmax'], f).to_grid(
+            st['beg'], st['end'], st['pos_v'], gd.h_cv, gd.n_c, gd.beg_c)
+        assert A.shape[0] == ref['rows']
+        assert digest(A) == ref['A_sha256'], 'l=%s' % l
+        assert G.tolist() == ref['G_B']
+
+
+def test_lfc_integrate_add():
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import projector_splines
+    lx = FX['lfc']
+    gd = grid((lx['N'],) * 3, True, lx['h_ang'])
+    pt = LFC(gd, [projector_splines(2)] * lx['natoms'])
+    pt.set_positions(np.random.default_rng(lx['spos_seed']).uniform(
+        0, 1, (lx['natoms'], 3)))
+    assert pt.Mmax == lx['Mmax'] and len(pt.A_Wgm) == lx['nW']
+    assert hashlib.sha256(pt.G_B.tobytes()).hexdigest() == lx['G_B_sha256']
+    rng = np.random.default_rng(lx['data_seed'])
+    a = rng.standard_normal((3, 40, 40, 40))
+    c = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a, pt.Mmax)
+    assert np.array_equal(c.ravel(), unhex(lx['integrate']))
+    cc = rng.standard_normal((3, pt.Mmax))
+    b = O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, cc, a)
+    assert digest(b) == lx['add_sha256']
+
+
+def test_preconditioner_sequence():
+    px = FX['preconditioner']
+    gd0 = grid((48, 48, 48), True, px['h_ang'])
+    gd1 = gd0.coarsen()
+    gd2 = gd1.coarsen()
+    c0, o0, _ = guc_laplace_stencil(gd0, -0.5, 3)
+    c1, o1, _ = guc_laplace_stencil(gd1, -0.5, 1)
+    c2, o2, _ = guc_laplace_stencil(gd2, -0.5, 1)
+    step = 0.66666666 / c0[0]
+    assert float(step).hex() == px['step']
+
+    def targs(gdin, gdout):
+        pad, npad, skip, interp = transformer_paddings(gdin, gdout, 1)
+        return (2, pad, skip, gdin.n_c, gdout.n_c)
+
+    R = rnd(px['seed'], (px['bands'], 48, 48, 48))
+    out = O.precondition((c0, o0, gd0.n_c), (c1, o1, gd1.n_c),
+                         (c2, o2, gd2.n_c), targs(gd0, gd1), targs(gd1, gd2),
+                         targs(gd2, gd1), targs(gd1, gd0), gd0.pbc_c, step, R)
+    assert digest(out) == px['out_sha256']

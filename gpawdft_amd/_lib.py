@@ -64,6 +64,15 @@ _SIGNATURES = {
                                 _vpp]),
     'gpd_precond_apply': (_i, [_vp, _i, _l, _vp, _vp, _vp]),
     'gpd_precond_destroy': (_i, [_vp]),
+    'gpd_inner_products': (_i, [_i, _i, _l, _vp, _vp, _d, _i, _vp, _l, _vp,
+                                _vp]),
+    'gpd_rotate': (_i, [_i, _i, _l, _vp, _vp, _vp, _vp]),
+    'gpd_paw_matrix_add': (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp]),
+    'gpd_inverse_cholesky': (_i, [_i, _vp, _c_int_p, _vp]),
+    'gpd_diagonalize': (_i, [_i, _vp, _vp, _c_int_p, _vp]),
+    'gpd_axpy_batched': (_i, [_i, _l, _l, _vp, _vp, _vp, _vp]),
+    'gpd_dots': (_i, [_i, _l, _l, _d, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

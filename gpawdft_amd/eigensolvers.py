@@ -1,0 +1,298 @@
+"""Eigensolver drivers over the device operators.
+
+Mirrors, for one k-point / spin on one GPU:
+* `Overlap.orthonormalize` (gpaw/overlap.py:29-101): projections, S,
+  inverse Cholesky, rotation;
+* `Eigensolver.calculate_residuals` / `.subspace_diagonalize` /
+  `.iterate` (gpaw/eigensolvers/eigensolver.py:50-172);
+* `RMMDIIS` (gpaw/eigensolvers/rmmdiis.py:10-344): block RMM-DIIS with
+  the preconditioned residual step, the lambda line search, DIIS over
+  `niter` vectors and the final trial step.
+
+The error measure is the reference's: sum_n w_n <R_n|R_n> (dv included).
+Every grid-sized operation runs on the device (stencil, projectors,
+preconditioner, MFMA subspace products, batched axpy/dots); only the
+per-band scalars (lambda, DIIS coefficients) pass through the host.
+"""
+import time
+
+import numpy as np
+
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr, torch
+from gpawdft_amd.hamiltonian import unpack
+from gpawdft_amd.hs_operators import BandLayouts, MatrixOperator
+
+
+def _nG(t):
+    return int(np.prod(t.shape[-3:]))
+
+
+def dots(a, b, alpha=1.0):
+    """alpha * <a_x|b_x> for each grid x (host FP64 array)."""
+    va = DeviceView(a)
+    vb = DeviceView(b)
+    nb = int(np.prod(a.shape[:-3], dtype=np.int64))
+    out = torch.empty(nb, dtype=torch.float64, device=va.t.device)
+    check(lib().gpd_dots(va.code, nb, _nG(a), float(alpha), va.ptr, vb.ptr,
+                         out.data_ptr(), stream_ptr()), 'dots')
+    return out.cpu().numpy()
+
+
+def axpy(coef_x, x, y):
+    """y_x += coef_x[x] * x_x for each grid x."""
+    vx = DeviceView(x)
+    vy = DeviceView(y)
+    nb = int(np.prod(x.shape[:-3], dtype=np.int64))
+    c = torch.as_tensor(np.broadcast_to(np.asarray(coef_x, float), (nb,))
+                        .copy(), device=vx.t.device)
+    check(lib().gpd_axpy_batched(vx.code, nb, _nG(x), c.data_ptr(), vx.ptr,
+                                 vy.ptr, stream_ptr()), 'axpy')
+
+
+class WaveFunctions:
+    """Grid wave functions of one k-point: operators and work arrays.
+
+    gd, kin (kinetic FDOperator), pt (projector LFC), nbands, dtype (torch
+    float32/float64); dO_aii: PAW overlap corrections (zeros by default)."""
+
+    def __init__(self, gd, kin, pt, nbands, dtype, dO_aii=None,
+                 preconditioner_factory=None):
+        self.gd = gd
+        self.kin = kin
+        self.pt = pt
+        self.nbands = nbands
+        self.dtype = dtype
+        self.dO_aii = dO_aii or {a: np.zeros((pt.get_function_count(a),) * 2)
+                                 for a in pt.atom_indices}
+        self.ksl = BandLayouts(gd, nbands)
+        self.matrixoperator = MatrixOperator(self.ksl)
+        self.overlap = Overlap()
+        self.orthonormalized = False
+        self._pc_factory = preconditioner_factory
+
+    def make_preconditioner(self, block=1):
+        if self._pc_factory is not None:
+            return self._pc_factory(self, block)
+        from gpawdft_amd.preconditioner import Preconditioner
+        return Preconditioner(self.gd, self.kin, float, block)
+
+    def empty(self, n):
+        return torch.empty((n,) + tuple(int(x) for x in self.gd.n_c),
+                           dtype=self.dtype, device='cuda')
+
+    def zeros_P(self, n):
+        return torch.zeros((n, self.pt.Mmax), dtype=self.dtype, device='cuda')
+
+    def apply_hamiltonian(self, ham, psit_xG, Htpsit_xG, P_xM=None):
+        """H psit (kinetic + local + projectors); returns the projections."""
+        return ham.apply_fused(psit_xG, Htpsit_xG, self.kin, self.pt,
+                               P_xM=P_xM)
+
+
+class KPoint:
+    def __init__(self, psit_nG, weight=1.0, f_n=None):
+        self.psit_nG = psit_nG
+        self.P_xM = None
+        self.eps_n = np.zeros(psit_nG.shape[0])
+        self.weight = weight
+        self.f_n = f_n
+        self.s = 0
+        self.q = -1
+
+
+class Overlap:
+    def orthonormalize(self, wfs, kpt, psit_nG=None):
+        """psit_nG <- L^-1 psit_nG with S = L L^T (gpaw/overlap.py:29-101);
+        the result goes to kpt.psit_nG."""
+        if psit_nG is None:
+            psit_nG = kpt.psit_nG
+        pt = wfs.pt
+        P = wfs.zeros_P(psit_nG.shape[0])
+        pt.integrate_xM(psit_nG, P)
+        op = wfs.matrixoperator
+        S = op.calculate_matrix_elements(psit_nG, P, lambda x: x,
+                                         wfs.dO_aii, pt=pt)
+        wfs.ksl.inverse_cholesky(S)
+        op.matrix_multiply(S, psit_nG, P, out_nG=kpt.psit_nG)
+        kpt.P_xM = P
+
+
+class Eigensolver:
+    def __init__(self, keep_htpsit=True, blocksize=1):
+        self.keep_htpsit = keep_htpsit
+        self.blocksize = blocksize
+        self.initialized = False
+        self.error = np.inf
+        self.Htpsit_nG = None
+
+    def initialize(self, wfs):
+        self.nbands = wfs.nbands
+        self.preconditioner = wfs.make_preconditioner(self.blocksize)
+        self.Htpsit_nG = wfs.empty(wfs.nbands)
+        self.initialized = True
+
+    def iterate(self, ham, wfs, kpt):
+        if not self.initialized:
+            self.initialize(wfs)
+        if not wfs.orthonormalized:
+            wfs.overlap.orthonormalize(wfs, kpt)
+            wfs.orthonormalized = True
+        error, psit_nG = self.iterate_one_k_point(ham, wfs, kpt)
+        wfs.overlap.orthonormalize(wfs, kpt, psit_nG)
+        self.error = error
+        return error
+
+    def calculate_residuals(self, kpt, wfs, ham, psit_xG, P_xM, eps_x, R_xG):
+        """R = H psit - eps psit + sum_a p_a (P (dH - eps dO))
+        (gpaw/eigensolvers/eigensolver.py:82-100); R_xG holds H psit on
+        entry."""
+        axpy(-np.asarray(eps_x), psit_xG, R_xG)
+        pt = wfs.pt
+        c = ham.projector_coefs(pt, P_xM, kpt.s)
+        if any(np.any(d) for d in wfs.dO_aii.values()):
+            Pe = P_xM * torch.as_tensor(eps_x, dtype=P_xM.dtype,
+                                        device=P_xM.device)[:, None]
+            for a in pt.atom_indices:
+                M1, M2 = pt.M_range(a)
+                dO = torch.as_tensor(wfs.dO_aii[a], dtype=P_xM.dtype,
+                                     device=P_xM.device)
+                c[:, M1:M2] -= Pe[:, M1:M2] @ dO
+        pt.add_xM(c, R_xG)
+
+    def subspace_diagonalize(self, ham, wfs, kpt):
+        """H_nn, eigenvalues, rotation of psit, P and H psit
+        (gpaw/eigensolvers/eigensolver.py:102-172)."""
+        psit_nG = kpt.psit_nG
+        Htpsit = self.Htpsit_nG
+
+        def H(psit_xG):
+            wfs.apply_hamiltonian(ham, psit_xG, Htpsit)
+            return Htpsit
+
+        dH = {a: unpack(ham.dH_asp[a][kpt.s]) for a in wfs.pt.atom_indices}
+        op = wfs.matrixoperator
+        H_nn = op.calculate_matrix_elements(psit_nG, kpt.P_xM, H, dH,
+                                            pt=wfs.pt)
+        wfs.ksl.diagonalize(H_nn, kpt.eps_n)
+        # rotated psit goes to a new array; rotated H psit reuses the
+        # storage of kpt.psit_nG, as the reference does
+        psit_nG = op.matrix_multiply(H_nn, psit_nG, kpt.P_xM)
+        Htpsit = op.matrix_multiply(H_nn, Htpsit, out_nG=kpt.psit_nG)
+        return psit_nG, Htpsit
+
+
+class RMMDIIS(Eigensolver):
+    def __init__(self, keep_htpsit=True, blocksize=10, niter=3, rtol=1e-16,
+                 limit_lambda=False, use_rayleigh=False, trial_step=0.1):
+        Eigensolver.__init__(self, keep_htpsit, blocksize)
+        if use_rayleigh:
+            raise NotImplementedError('use_rayleigh is not part of this build')
+        self.niter = niter
+        self.rtol = rtol
+        self.limit_lambda = limit_lambda
+        self.trial_step = trial_step
+        self.timings = {}
+
+    def _tick(self, key, t0):
+        torch.cuda.synchronize()
+        self.timings[key] = self.timings.get(key, 0.0) + time.perf_counter() - t0
+
+    def iterate_one_k_point(self, ham, wfs, kpt):
+        """One RMM-DIIS sweep over all bands (gpaw/eigensolvers/
+        rmmdiis.py:52-344).  Returns (error, psit_nG)."""
+        t0 = time.perf_counter()
+        psit_nG, R_nG = self.subspace_diagonalize(ham, wfs, kpt)
+        self._tick('subspace', t0)
+        t0 = time.perf_counter()
+        self.calculate_residuals(kpt, wfs, ham, psit_nG, kpt.P_xM,
+                                 kpt.eps_n, R_nG)
+        dv = wfs.gd.dv
+        B = self.blocksize
+        niter = self.niter
+        nb = wfs.nbands
+        dR_xG = wfs.empty(B)
+        psit_diis = wfs.empty(B * niter) if niter > 1 else None
+        R_diis = wfs.empty(B * niter) if niter > 1 else None
+        P_d = wfs.zeros_P(B)
+        error = 0.0
+        pc = self.preconditioner
+        for n1 in range(0, nb, B):
+            n2 = min(n1 + B, nb)
+            Bx = n2 - n1
+            psit_x = psit_nG[n1:n2]
+            R_x = R_nG[n1:n2]
+            dR_x = dR_xG[:Bx]
+            eps_x = kpt.eps_n[n1:n2]
+            if kpt.f_n is None:
+                w_x = np.full(Bx, kpt.weight)
+            else:
+                w_x = np.asarray(kpt.f_n[n1:n2])
+            error += float(np.sum(w_x * dots(R_x, R_x, dv)))
+            if niter > 1:
+                psit_diis.view(B, niter, *psit_x.shape[1:])[:Bx, 0] = psit_x
+                R_diis.view(B, niter, *R_x.shape[1:])[:Bx, 0] = R_x
+            dpsit_x = pc(R_x, kpt)
+            P_x = P_d[:Bx]
+            wfs.apply_hamiltonian(ham, dpsit_x, dR_x, P_x)
+            self.calculate_residuals(kpt, wfs, ham, dpsit_x, P_x, eps_x,
+                                     dR_x)
+            RdR = dots(dR_x, R_x, dv)
+            dRdR = dots(dR_x, dR_x, dv)
+            lam_x = -RdR / dRdR
+            if self.limit_lambda:
+                upper = self.limit_lambda['upper']
+                lower = self.limit_lambda['lower']
+                if self.limit_lambda.get('absolute', False):
+                    lam_x = np.where(np.abs(lam_x) < lower,
+                                     lower * np.sign(lam_x), lam_x)
+                    lam_x = np.where(np.abs(lam_x) > upper,
+                                     upper * np.sign(lam_x), lam_x)
+                else:
+                    lam_x = np.clip(lam_x, lower, upper)
+            axpy(lam_x, dpsit_x, psit_x)
+            axpy(lam_x, dR_x, R_x)
+            for nit in range(1, niter):
+                pd = psit_diis.view(B, niter, *psit_x.shape[1:])
+                rd = R_diis.view(B, niter, *R_x.shape[1:])
+                pd[:Bx, nit] = psit_x
+                rd[:Bx, nit] = R_x
+                if nit > 1 or self.limit_lambda:
+                    self._diis(pd[:Bx], rd[:Bx], nit, psit_x, R_x, dv)
+                if nit < niter - 1:
+                    dpsit_x = pc(R_x, kpt)
+                    axpy(lam_x, dpsit_x, psit_x)
+                    wfs.apply_hamiltonian(ham, psit_x, R_x, P_x)
+                    self.calculate_residuals(kpt, wfs, ham, psit_x, P_x,
+                                             eps_x, R_x)
+            dpsit_x = pc(R_x, kpt)
+            if self.trial_step is not None:
+                lam_x = np.full(Bx, self.trial_step)
+            axpy(lam_x, dpsit_x, psit_x)
+        self._tick('rmm-diis', t0)
+        return error, psit_nG
+
+    def _diis(self, pd, rd, nit, psit_x, R_x, dv):
+        """Per band: minimise |sum_i a_i R_i| with sum_i a_i = 1 over the
+        nit + 1 stored residuals, then psit = sum_i a_i psit_i (and R)."""
+        Bx = pd.shape[0]
+        m = nit + 1
+        R_bnn = np.empty((Bx, m, m))
+        for i in range(m):
+            for j in range(i, m):
+                v = dots(rd[:, i], rd[:, j], dv)
+                R_bnn[:, i, j] = v
+                R_bnn[:, j, i] = v
+        alpha = np.empty((Bx, m))
+        for ib in range(Bx):
+            A = -np.ones((m + 1, m + 1))
+            A[:m, :m] = R_bnn[ib]
+            A[-1, -1] = 0.0
+            x = np.zeros(m + 1)
+            x[-1] = -1.0
+            alpha[ib] = np.linalg.solve(A, x)[:-1]
+        psit_x.zero_()
+        R_x.zero_()
+        for i in range(m):
+            axpy(alpha[:, i], pd[:, i].contiguous(), psit_x)
+            axpy(alpha[:, i], rd[:, i].contiguous(), R_x)

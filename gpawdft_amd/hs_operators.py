@@ -1,0 +1,221 @@
+"""Subspace matrices, rotations and the dense FP64 steps, on the device.
+
+Drop-ins for the serial ("simple case", one band group, one block) paths
+of the reference:
+* `MatrixOperator(ksl, nblocks=None, async_=None, hermitian=None)` with
+  `.calculate_matrix_elements(psit1_nG, P1_ani, A, dA, psit2_nG=None,
+  P2_ani=None)` and `.matrix_multiply(C_NN, psit_nG, P_ani=None,
+  out_nG=None)` (gpaw/hs_operators.py:31, 251-313, 417-466).  The
+  reference's keyword `async` is a reserved word in Python 3; it is
+  `async_` here.
+* `BandLayouts(gd, nbands, dtype)` with `.diagonalize(H_NN, eps_n)` and
+  `.inverse_cholesky(S_NN)` (gpaw/kohnsham_layouts.py:112-162).
+* `diagonalize(a, w)` / `inverse_cholesky(a)` (gpaw/utilities/lapack.py:
+  18-36, 109-123): FP64 on the GPU, RuntimeError on a non-zero info.
+* `grid_inner_products`: GridDescriptor.integrate(a, b) for arrays.
+
+Matrices are FP64 device tensors (N, N); both triangles are filled (the
+reference fills the lower one, which is all its consumers read).  Grid
+arrays may be FP32 (split-bfloat16 MFMA products) or FP64 (FP64 MFMA).
+"""
+import numpy as np
+
+from gpawdft_amd import _lib
+from gpawdft_amd._lib import check, lib
+from gpawdft_amd.device import DeviceView, stream_ptr, to_device, torch
+
+
+def _flat(v):
+    """(..., n0, n1, n2) device tensor -> (N, G) view."""
+    t = v.t
+    nG = int(np.prod(t.shape[-3:]))
+    return t.reshape(-1, nG), nG
+
+
+def grid_inner_products(gd, a_xg, b_yg, hermitian=False,
+                        _transposed_result=None, alpha=None):
+    """dv * <a_x | b_y> as an (Nx, Ny) FP64 device matrix (Nx == Ny).
+
+    Same conventions as GridDescriptor.integrate: `a_xg is b_yg` gives the
+    overlap-type product, `hermitian` symmetrises; the result (both
+    triangles) is written to `_transposed_result` if given."""
+    va = DeviceView(a_xg)
+    vb = va if b_yg is a_xg else DeviceView(b_yg)
+    A, K = _flat(va)
+    B, _ = _flat(vb)
+    N = A.shape[0]
+    if B.shape[0] != N:
+        raise NotImplementedError('rectangular inner products are not '
+                                  'needed by this path')
+    out = _transposed_result
+    if out is None:
+        out = torch.empty((N, N), dtype=torch.float64, device=A.device)
+    alpha = gd.dv if alpha is None else alpha
+    herm = 1 if (hermitian and b_yg is not a_xg) else 0
+    check(lib().gpd_inner_products(va.code, N, K, A.data_ptr(), B.data_ptr(),
+                                   float(alpha), herm, None, 0,
+                                   out.data_ptr(), stream_ptr()),
+          'inner_products')
+    return out
+
+
+def rotate(C_NN, in_nG, out_nG):
+    """out_nG = C_NN in_nG (band rotation) on the device."""
+    vin = DeviceView(in_nG)
+    vout = DeviceView(out_nG, copy_in=False)
+    X, K = _flat(vin)
+    N = X.shape[0]
+    C = C_NN if isinstance(C_NN, torch.Tensor) else to_device(C_NN)
+    C = C.to(torch.float64).contiguous()
+    if C.shape != (N, N):
+        raise ValueError('C_NN must be %d x %d' % (N, N))
+    check(lib().gpd_rotate(vin.code, N, K, C.data_ptr(), X.data_ptr(),
+                           vout.ptr, stream_ptr()), 'rotate')
+    vout.finish()
+    return out_nG
+
+
+def inverse_cholesky(a):
+    """a (FP64 device (n, n), lower triangle) <- L^-1, upper zeroed."""
+    if not (isinstance(a, torch.Tensor) and a.is_cuda and
+            a.dtype == torch.float64 and a.is_contiguous()):
+        raise TypeError('inverse_cholesky needs a contiguous FP64 cuda tensor')
+    n = a.shape[0]
+    assert a.shape == (n, n)
+    info = _lib.ctypes.c_int()
+    check(lib().gpd_inverse_cholesky(n, a.data_ptr(), _lib.ctypes.byref(info),
+                                     stream_ptr()), 'inverse_cholesky')
+    if info.value != 0:
+        raise RuntimeError('inverse_cholesky error: %d' % info.value)
+
+
+def diagonalize(a, w):
+    """Eigenvalues of the symmetric a (lower triangle) into w (ascending),
+    eigenvectors into the rows of a."""
+    for x in (a, w):
+        if not (isinstance(x, torch.Tensor) and x.is_cuda and
+                x.dtype == torch.float64 and x.is_contiguous()):
+            raise TypeError('diagonalize needs contiguous FP64 cuda tensors')
+    n = a.shape[0]
+    assert a.shape == (n, n) and w.shape == (n,)
+    sweeps = _lib.ctypes.c_int()
+    check(lib().gpd_diagonalize(n, a.data_ptr(), w.data_ptr(),
+                                _lib.ctypes.byref(sweeps), stream_ptr()),
+          'diagonalize')
+    return sweeps.value
+
+
+class BandLayouts:
+    """Serial (one GPU) layouts: the dense FP64 steps on the device."""
+
+    def __init__(self, gd, nbands, dtype=float, timer=None):
+        self.gd = gd
+        self.nbands = nbands
+        self.dtype = dtype
+        self.last_sweeps = 0
+
+    def diagonalize(self, H_NN, eps_n):
+        w = torch.empty(self.nbands, dtype=torch.float64, device=H_NN.device)
+        self.last_sweeps = diagonalize(H_NN, w)
+        if isinstance(eps_n, np.ndarray):
+            eps_n[:] = w.cpu().numpy()
+        else:
+            eps_n.copy_(w)
+
+    def inverse_cholesky(self, S_NN):
+        inverse_cholesky(S_NN)
+
+    def get_description(self):
+        return 'Serial GPU (Jacobi eigensolver, inverse Cholesky)'
+
+
+def _dA_tables(pt, dA, device):
+    """(natoms, M_a, ni_a, offsets, blocks) for gpd_paw_matrix_add from a
+    dict {a: dA_ii} or a callable dA(a, P_ni) = P_ni dA_ii (evaluated on
+    the identity to read dA_ii)."""
+    M_a, ni_a, off, blocks = [], [], [], []
+    o = 0
+    for a in pt.atom_indices:
+        ni = pt.get_function_count(a)
+        if callable(dA):
+            D = dA(a, np.eye(ni))
+            D = D.cpu().numpy() if isinstance(D, torch.Tensor) else np.asarray(D)
+        else:
+            D = np.asarray(dA[a])
+        D = np.ascontiguousarray(D, float).reshape(ni, ni)
+        M_a.append(pt.M_a[a])
+        ni_a.append(ni)
+        off.append(o)
+        o += ni * ni
+        blocks.append(D.ravel())
+    dev = lambda x, dt: to_device(np.asarray(x, dt)).to(device)
+    return (len(M_a), dev(M_a, np.intc), dev(ni_a, np.intc),
+            dev(off, np.intc),
+            dev(np.concatenate(blocks) if blocks else np.zeros(1), float))
+
+
+class MatrixOperator:
+    nblocks = 1
+    async_ = True
+    hermitian = True
+
+    def __init__(self, ksl, nblocks=None, async_=None, hermitian=None):
+        self.ksl = ksl
+        self.gd = ksl.gd
+        if nblocks not in (None, 1):
+            raise NotImplementedError('blocked / band-parallel paths are not '
+                                      'part of this one-GPU build')
+        if hermitian is not None:
+            self.hermitian = hermitian
+        self.A_nn = None
+
+    def calculate_matrix_elements(self, psit1_nG, P1_ani, A, dA,
+                                  psit2_nG=None, P2_ani=None, pt=None):
+        """A_NN = dv <psi1|A|psi2> + sum_a P1 dA_a P2^T (FP64 device).
+
+        P1_ani / P2_ani: a (nbands, Mmax) device tensor of projections, or
+        None for no PAW term; `pt` (the LFC) locates the atoms' columns."""
+        if psit2_nG is None:
+            psit2_nG = psit1_nG
+            hermitian = self.hermitian
+        else:
+            hermitian = False
+        if P2_ani is None:
+            P2_ani = P1_ani
+        Apsit = A(psit2_nG)
+        N = psit1_nG.shape[0]
+        dev = psit1_nG.device
+        if self.A_nn is None or self.A_nn.shape != (N, N):
+            self.A_nn = torch.empty((N, N), dtype=torch.float64, device=dev)
+        same = Apsit is psit1_nG
+        grid_inner_products(self.gd, psit1_nG, psit1_nG if same else Apsit,
+                            hermitian=hermitian, _transposed_result=self.A_nn)
+        if P1_ani is not None and pt is not None and pt.Mmax:
+            natoms, M_a, ni_a, off, D = _dA_tables(pt, dA, dev)
+            code = DeviceView(P1_ani).code
+            check(lib().gpd_paw_matrix_add(
+                code, N, int(pt.Mmax), natoms, M_a.data_ptr(),
+                ni_a.data_ptr(), off.data_ptr(), D.data_ptr(),
+                P1_ani.data_ptr(), P2_ani.data_ptr(), self.A_nn.data_ptr(),
+                stream_ptr()), 'paw_matrix_add')
+        return self.A_nn
+
+    def matrix_multiply(self, C_NN, psit_nG, P_ani=None, out_nG=None):
+        """psit_n <- sum_m C_nm psit_m (and the projections likewise)."""
+        if out_nG is None or out_nG is psit_nG:
+            work = torch.empty_like(psit_nG)
+            rotate(C_NN, psit_nG, work)
+            if out_nG is psit_nG:
+                psit_nG.copy_(work)
+                out_nG = psit_nG
+            else:
+                out_nG = work
+        else:
+            rotate(C_NN, psit_nG, out_nG)
+        if P_ani is not None:
+            P = P_ani
+            tmp = torch.empty_like(P)
+            rotate(C_NN, P.reshape(P.shape[0], 1, 1, P.shape[1]),
+                   tmp.reshape(P.shape[0], 1, 1, P.shape[1]))
+            P.copy_(tmp)
+        return out_nG

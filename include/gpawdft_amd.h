@@ -184,6 +184,66 @@ int gpd_precond_apply(gpd_precond* pc, int dtype, long nb, const void* d_R,
 
 int gpd_precond_destroy(gpd_precond* pc);
 
+/* ------------------------------------------------------------------ */
+/* Subspace matrices and rotations (MatrixOperator, gpaw/hs_operators.py).
+ *
+ * gpd_inner_products: out[i][j] = alpha * sum_k X[i][k] Y[j][k] for i, j <
+ * N (full N x N, FP64, row-major); hermitian != 0 gives
+ * 0.5 alpha (T + T^T) -- the rk / r2k calls of GridDescriptor.integrate
+ * (gpaw/grid_descriptor.py:300-306, c/blas.c:290-353), both triangles
+ * filled.  X, Y: (N, K) of dtype.  FP32 runs split-bfloat16 MFMA (hi+lo
+ * per operand, four products, FP32 accumulation per K-chunk, FP64 sum of
+ * the chunks); FP64 runs FP64 MFMA.  d_work (may be NULL) is device
+ * scratch for the per-chunk partials; if too small, the call allocates.  */
+int gpd_inner_products(int dtype, int N, long K, const void* d_X,
+                       const void* d_Y, double alpha, int hermitian,
+                       void* d_work, long work_bytes, double* d_out,
+                       void* stream);
+
+/* gpd_rotate: Y[n][k] = sum_m C[n][m] X[m][k] -- MatrixOperator.
+ * matrix_multiply / GridDescriptor.gemm (gpaw/hs_operators.py:453-466,
+ * c/blas.c:95-145).  C: FP64 (N, N) row-major; X, Y: (N, K) of dtype,
+ * distinct.                                                             */
+int gpd_rotate(int dtype, int N, long K, const double* d_C, const void* d_X,
+               void* d_Y, void* stream);
+
+/* gpd_paw_matrix_add: A[i][j] += sum_a (P2 dA_a P1^T)[i][j] over atoms
+ * (gemm(1.0, P1_ni, dA(a, P2_ni), 1.0, A_NN, 'c'), gpaw/hs_operators.py:
+ * 309-311).  P1, P2: (N, Mmax) of dtype; dA blocks FP64 row-major ni x ni
+ * at dA_off[a]; A: FP64 (N, N).                                         */
+int gpd_paw_matrix_add(int dtype, int N, int Mmax, int natoms,
+                       const int* d_M_a, const int* d_ni_a,
+                       const int* d_dA_off, const double* d_dA,
+                       const void* d_P1, const void* d_P2, double* d_A,
+                       void* stream);
+
+/* gpd_inverse_cholesky: inverse_cholesky(a) (c/lapack.c:324-370): the
+ * lower triangle of the FP64 (n, n) matrix holds S; on return it holds
+ * L^-1 with S = L L^T and the upper triangle is zero.  *info > 0 (LAPACK
+ * dpotrf convention) if S is not positive definite.  Synchronises the
+ * stream.                                                               */
+int gpd_inverse_cholesky(int n, double* d_A, int* info, void* stream);
+
+/* gpd_diagonalize: diagonalize(a, w) (c/lapack.c:129-160): eigenvalues of
+ * the symmetric FP64 (n, n) matrix whose lower triangle is given, in
+ * ascending order into d_w; eigenvectors into the rows of d_A.  Jacobi
+ * sweeps; *sweeps receives their number (may be NULL).  Synchronises the
+ * stream once per sweep.                                                */
+int gpd_diagonalize(int n, double* d_A, double* d_w, int* sweeps,
+                    void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Band-wise vector operations of the eigensolver (the per-band BLAS axpy
+ * and dot calls of gpaw/eigensolvers/rmmdiis.py:107-122, 199-228 and
+ * eigensolver.py:88-89), batched over nb grids of nG points.
+ * gpd_axpy_batched: y_x += coef[x] * x_x  (coef: FP64 device, nb).
+ * gpd_dots: out[x] = alpha * sum_G a_x[G] b_x[G], FP64 accumulation in a
+ * fixed order (out: FP64 device, nb <= 65535).                          */
+int gpd_axpy_batched(int dtype, long nb, long nG, const double* d_coef,
+                     const void* d_x, void* d_y, void* stream);
+int gpd_dots(int dtype, long nb, long nG, double alpha, const void* d_a,
+             const void* d_b, double* d_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,171 @@
+"""GPU subspace step: inner products (split-bfloat16 MFMA for FP32, FP64
+MFMA), rotations, PAW term, FP64 inverse Cholesky and Jacobi eigensolver,
+against numpy/LAPACK FP64 (the reference calls BLAS/LAPACK here, whose bit
+pattern is not pinned; the tolerances are written per test)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def smooth_orbitals(N, n_c, seed):
+    from gpawdft_amd.synthetic import smooth_bands_numpy
+    return smooth_bands_numpy(N, n_c, seed)
+
+
+@pytest.mark.parametrize('N,n', [(64, 24), (100, 20), (33, 16)])
+def test_inner_products_f64(cuda, N, n):
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.hs_operators import grid_inner_products
+    gd = GridDescriptor((n, n, n), (5.0, 5.0, 5.0))
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((N, n, n, n))
+    y = rng.standard_normal((N, n, n, n))
+    X, Y = x.reshape(N, -1), y.reshape(N, -1)
+    S = grid_inner_products(gd, dev(x), dev(x)).cpu().numpy()
+    ref = gd.dv * X @ X.T
+    assert np.abs(S - ref).max() < 1e-12 * np.abs(ref).max()
+    H = grid_inner_products(gd, dev(x), dev(y), hermitian=True).cpu().numpy()
+    T = gd.dv * X @ Y.T
+    ref = 0.5 * (T + T.T)
+    assert np.abs(H - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('N,n', [(64, 32), (96, 24), (40, 20)])
+def test_inner_products_split_bf16(cuda, N, n):
+    """FP32 grids: S and H from split-bfloat16 MFMA within 1e-6 (relative
+    Frobenius) of the FP64 products of the same FP32 data."""
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.hs_operators import grid_inner_products
+    gd = GridDescriptor((n, n, n), (5.0, 5.0, 5.0))
+    x = smooth_orbitals(N, gd.n_c, 2).astype(np.float32)
+    y = (np.random.default_rng(3).standard_normal(x.shape)).astype(np.float32)
+    X, Y = x.reshape(N, -1).astype(np.float64), y.reshape(N, -1).astype(np.float64)
+    S = grid_inner_products(gd, dev(x), dev(x)).cpu().numpy()
+    ref = gd.dv * X @ X.T
+    assert np.linalg.norm(S - ref) < 1e-6 * np.linalg.norm(ref)
+    H = grid_inner_products(gd, dev(x), dev(y), hermitian=True).cpu().numpy()
+    T = gd.dv * X @ Y.T
+    ref = 0.5 * (T + T.T)
+    assert np.linalg.norm(H - ref) < 1e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize('dtype,tol', [(np.float64, 1e-13), (np.float32, 1e-6)])
+def test_rotate(cuda, dtype, tol):
+    from gpawdft_amd.hs_operators import rotate
+    N, n = 70, 20
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((N, n, n, n)).astype(dtype)
+    C = rng.standard_normal((N, N))
+    import torch
+    out = torch.empty(x.shape, dtype=torch.float64 if dtype == np.float64
+                      else torch.float32, device='cuda')
+    rotate(dev(C), dev(x), out)
+    ref = (C @ x.reshape(N, -1).astype(np.float64)).reshape(x.shape)
+    got = out.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(got - ref) < tol * np.linalg.norm(ref)
+
+
+def test_inverse_cholesky(cuda):
+    import torch
+    from gpawdft_amd.hs_operators import inverse_cholesky
+    rng = np.random.default_rng(5)
+    for n in (1, 7, 64, 200):
+        X = rng.standard_normal((n, n + 5))
+        S = X @ X.T / (n + 5) + 0.1 * np.eye(n)
+        a = dev(np.tril(S) + np.triu(np.full((n, n), 7.0), 1))  # upper ignored
+        inverse_cholesky(a)
+        C = a.cpu().numpy()
+        assert np.abs(np.triu(C, 1)).max() == 0.0
+        L = np.linalg.cholesky(S)
+        assert np.abs(C - np.linalg.inv(L)).max() < 1e-10 * np.abs(C).max()
+        assert np.abs(C @ S @ C.T - np.eye(n)).max() < 1e-12 * n
+    bad = dev(np.diag([1.0, -2.0, 3.0]))
+    with pytest.raises(RuntimeError):
+        inverse_cholesky(bad)
+
+
+def test_diagonalize(cuda):
+    import torch
+    from gpawdft_amd.hs_operators import diagonalize
+    rng = np.random.default_rng(6)
+    for n in (1, 2, 5, 64, 129):
+        X = rng.standard_normal((n, n))
+        H = X + X.T
+        H[np.diag_indices(n)] += np.linspace(-3, 3, n)
+        a = dev(np.tril(H) + np.triu(np.full((n, n), -9.0), 1))
+        w = torch.empty(n, dtype=torch.float64, device='cuda')
+        diagonalize(a, w)
+        e, U = np.linalg.eigh(H)
+        W = w.cpu().numpy()
+        V = a.cpu().numpy()     # eigenvectors in rows
+        assert np.abs(W - e).max() < 1e-12 * max(1, np.abs(e).max())
+        assert (np.diff(W) >= 0).all()
+        assert np.abs(V @ H @ V.T - np.diag(W)).max() < 1e-11 * np.abs(e).max()
+        assert np.abs(V @ V.T - np.eye(n)).max() < 1e-12 * n
+
+
+def test_matrix_operator_orthonormalize_and_diag(cuda):
+    """The reference's two subspace steps on a small PAW system:
+    orthonormalize (S -> inverse Cholesky -> rotate) gives S' = 1, and
+    subspace diagonalisation gives a diagonal H' with the eigenvalues of H
+    (gpaw/test/parallel/overlap.py, hamiltonian.py invariants)."""
+    import torch
+    from gpawdft_amd.hamiltonian import Hamiltonian
+    from gpawdft_amd.hs_operators import BandLayouts, MatrixOperator
+    from gpawdft_amd.synthetic import System
+    sysm = System(24, 0.18, 3, seed=7)
+    kin, pt = sysm.operators()
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    N = 20
+    psi = dev(smooth_orbitals(N, sysm.gd.n_c, 8))
+    rng = np.random.default_rng(9)
+    dO = {}
+    for a in range(3):
+        X = 0.02 * rng.standard_normal((9, 9))
+        dO[a] = X + X.T
+    ksl = BandLayouts(sysm.gd, N)
+    op = MatrixOperator(ksl)
+    P = torch.zeros((N, pt.Mmax), dtype=torch.float64, device='cuda')
+    pt.integrate_xM(psi, P)
+    S = op.calculate_matrix_elements(psi, P, lambda x: x,
+                                     lambda a, P_ni: P_ni @ dO[a], pt=pt)
+    # reference S on the host
+    x = psi.cpu().numpy().reshape(N, -1)
+    Ph = P.cpu().numpy()
+    Sref = sysm.gd.dv * x @ x.T
+    for a in range(3):
+        M1, M2 = pt.M_range(a)
+        Sref += Ph[:, M1:M2] @ dO[a] @ Ph[:, M1:M2].T
+    assert np.abs(S.cpu().numpy() - Sref).max() < 1e-12 * np.abs(Sref).max()
+    ksl.inverse_cholesky(S)
+    op.matrix_multiply(S, psi, P, out_nG=psi)
+    # orthonormal now
+    P2 = torch.zeros_like(P)
+    pt.integrate_xM(psi, P2)
+    assert torch.allclose(P2, P, rtol=0, atol=1e-12)
+    S2 = op.calculate_matrix_elements(psi, P, lambda x: x,
+                                      lambda a, P_ni: P_ni @ dO[a], pt=pt)
+    assert np.abs(S2.cpu().numpy() - np.eye(N)).max() < 1e-10
+    # subspace diagonalisation (no dO in H)
+    Hpsi = torch.empty_like(psi)
+
+    def Hop(p):
+        ham.apply_fused(p, Hpsi, kin, pt)
+        return Hpsi
+    from gpawdft_amd.hamiltonian import unpack
+    dH = {a: unpack(sysm.dH_asp[a][0]) for a in range(3)}
+    H = op.calculate_matrix_elements(psi, P, Hop, dH, pt=pt)
+    Hh = H.cpu().numpy()
+    e_ref = np.linalg.eigvalsh(np.tril(Hh) + np.tril(Hh, -1).T)
+    eps = np.empty(N)
+    ksl.diagonalize(H, eps)
+    assert np.abs(eps - e_ref).max() < 1e-10 * np.abs(e_ref).max()
+    op.matrix_multiply(H, psi, P, out_nG=psi)
+    H2 = op.calculate_matrix_elements(psi, P, Hop, dH, pt=pt).cpu().numpy()
+    assert np.abs(H2 - np.diag(eps)).max() < 1e-8 * np.abs(eps).max()

@@ -28,6 +28,7 @@ _vpp = ctypes.POINTER(ctypes.c_void_p)
 _i = ctypes.c_int
 _l = ctypes.c_long
 _d = ctypes.c_double
+_f = ctypes.c_float
 
 _SIGNATURES = {
     'gpd_last_error': (ctypes.c_char_p, []),
@@ -73,6 +74,16 @@ _SIGNATURES = {
     'gpd_diagonalize': (_i, [_i, _vp, _vp, _c_int_p, _vp]),
     'gpd_axpy_batched': (_i, [_i, _l, _l, _vp, _vp, _vp, _vp]),
     'gpd_dots': (_i, [_i, _l, _l, _d, _vp, _vp, _vp, _vp]),
+    'gpd_conv3d_forward': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
+                                _i, _vp]),
+    'gpd_conv3d_backward_data': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp,
+                                      _i, _vp]),
+    'gpd_conv3d_backward_weight': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp,
+                                        _vp, _vp, _vp]),
+    'gpd_leaky_relu': (_i, [_l, _f, _vp, _vp, _vp]),
+    'gpd_leaky_relu_backward': (_i, [_l, _f, _vp, _vp, _vp, _vp]),
+    'gpd_scale_add': (_i, [_l, _f, _vp, _f, _vp, _vp]),
+    'gpd_adam_step': (_i, [_l, _f, _f, _f, _f, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

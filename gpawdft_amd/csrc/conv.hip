@@ -1,0 +1,282 @@
+// 3-D convolution kernels of the learned preconditioner (gfx950).
+//
+// Periodic 3x3x3 convolutions on grids (B, C, D, H, W), C-contiguous, the
+// layout of a batch of band grids with a channel axis.  No counterpart in
+// the reference: these serve the learned correction that stands beside the
+// multigrid preconditioner.
+//
+//   forward:        y[b,o,p] = bias[o] + sum_{i,t} W[o,i,t] x[b,i,p+d_t]
+//   backward data:  gx[b,i,p] = sum_{o,t} W[o,i,t] gy[b,o,p-d_t]
+//   backward weight gW[o,i,t] = sum_{b,p} gy[b,o,p] x[b,i,p+d_t]
+//   backward bias   gb[o]     = sum_{b,p} gy[b,o,p]
+// with d_t = (t/9 - 1, (t/3)%3 - 1, t%3 - 1) and periodic wrap.  Weights are
+// FP32 (o, i, 27).  Each thread of the data kernels computes one point for
+// 16 output channels; the weight kernel gives each (o, i) pair a workgroup
+// that reduces its 27 sums in LDS (fixed order, no atomics).  FP32 with FMA.
+#include "common.hpp"
+
+namespace gpd {
+
+constexpr int kCOB = 16;   // output channels per thread
+
+struct ConvDims {
+    int B, Cin, Cout, D, H, W;
+};
+
+__device__ __forceinline__ int wrapp(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// mode 0: forward (weight index o*Cin*27 + i*27 + t); mode 1: backward data
+// (roles of o and i swapped, taps mirrored).
+template <int MODE>
+__global__ __launch_bounds__(256) void conv3d_kernel(const float* __restrict__ x,
+                                                    const float* __restrict__ Wt,
+                                                    const float* __restrict__ bias,
+                                                    float* __restrict__ y, ConvDims d,
+                                                    int accumulate)
+{
+    const int P = d.D * d.H * d.W;
+    // channels seen by this kernel: nin inputs, nout outputs
+    const int nin = MODE == 0 ? d.Cin : d.Cout;
+    const int nout = MODE == 0 ? d.Cout : d.Cin;
+    const int ocb = (nout + kCOB - 1) / kCOB;
+    const long w = blockIdx.y;                  // (b, output-channel block)
+    const int b = (int)(w / ocb), o0 = (int)(w % ocb) * kCOB;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int px = p % d.W, py = (p / d.W) % d.H, pz = p / (d.W * d.H);
+    int nb[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        const int s = MODE == 0 ? 1 : -1;
+        const int z = wrapp(pz + s * (t / 9 - 1), d.D);
+        const int yy = wrapp(py + s * ((t / 3) % 3 - 1), d.H);
+        const int xx = wrapp(px + s * (t % 3 - 1), d.W);
+        nb[t] = (z * d.H + yy) * d.W + xx;
+    }
+    float acc[kCOB];
+#pragma unroll
+    for (int c = 0; c < kCOB; c++) acc[c] = 0.f;
+    const float* xb = x + (long)b * nin * P;
+    for (int i = 0; i < nin; i++) {
+        const float* xi = xb + (long)i * P;
+        float v[27];
+#pragma unroll
+        for (int t = 0; t < 27; t++) v[t] = xi[nb[t]];
+#pragma unroll
+        for (int c = 0; c < kCOB; c++) {
+            const int o = o0 + c;
+            if (o >= nout) break;
+            const float* wr = MODE == 0 ? Wt + ((long)o * d.Cin + i) * 27
+                                        : Wt + ((long)i * d.Cin + o) * 27;
+            float s = acc[c];
+#pragma unroll
+            for (int t = 0; t < 27; t++) s = __builtin_fmaf(wr[t], v[t], s);
+            acc[c] = s;
+        }
+    }
+    float* yb = y + (long)b * nout * P;
+#pragma unroll
+    for (int c = 0; c < kCOB; c++) {
+        const int o = o0 + c;
+        if (o >= nout) break;
+        float r = acc[c] + ((MODE == 0 && bias) ? bias[o] : 0.f);
+        float* dst = yb + (long)o * P + p;
+        *dst = accumulate ? *dst + r : r;
+    }
+}
+
+// gW[o,i,t] = sum_{b,p} gy[b,o,p] x[b,i,p+d_t]; one workgroup per (o, i)
+__global__ __launch_bounds__(256) void conv3d_wgrad_kernel(const float* __restrict__ x,
+                                                          const float* __restrict__ gy,
+                                                          float* __restrict__ gW, ConvDims d)
+{
+    __shared__ float red[256][28];
+    const int P = d.D * d.H * d.W;
+    const int o = blockIdx.x / d.Cin, i = blockIdx.x % d.Cin;
+    float acc[27];
+#pragma unroll
+    for (int t = 0; t < 27; t++) acc[t] = 0.f;
+    for (int b = 0; b < d.B; b++) {
+        const float* xi = x + ((long)b * d.Cin + i) * P;
+        const float* go = gy + ((long)b * d.Cout + o) * P;
+        for (int p = threadIdx.x; p < P; p += blockDim.x) {
+            const float g = go[p];
+            const int px = p % d.W, py = (p / d.W) % d.H, pz = p / (d.W * d.H);
+#pragma unroll
+            for (int t = 0; t < 27; t++) {
+                const int z = wrapp(pz + t / 9 - 1, d.D);
+                const int yy = wrapp(py + (t / 3) % 3 - 1, d.H);
+                const int xx = wrapp(px + t % 3 - 1, d.W);
+                acc[t] = __builtin_fmaf(g, xi[(z * d.H + yy) * d.W + xx], acc[t]);
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 27; t++) red[threadIdx.x][t] = acc[t];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int t = 0; t < 27; t++) red[threadIdx.x][t] += red[threadIdx.x + s][t];
+        __syncthreads();
+    }
+    if (threadIdx.x < 27) gW[((long)o * d.Cin + i) * 27 + threadIdx.x] = red[0][threadIdx.x];
+}
+
+// gb[o] = sum_{b,p} gy[b,o,p]; one workgroup per o
+__global__ __launch_bounds__(256) void conv3d_bgrad_kernel(const float* __restrict__ gy,
+                                                          float* __restrict__ gb, ConvDims d)
+{
+    __shared__ float red[256];
+    const int P = d.D * d.H * d.W;
+    const int o = blockIdx.x;
+    float s = 0.f;
+    for (int b = 0; b < d.B; b++) {
+        const float* go = gy + ((long)b * d.Cout + o) * P;
+        for (int p = threadIdx.x; p < P; p += blockDim.x) s += go[p];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) gb[o] = red[0];
+}
+
+// leaky ReLU and its gradient (slope a for x < 0), elementwise
+__global__ void leaky_relu_kernel(long n, float a, const float* __restrict__ x,
+                                  float* __restrict__ y)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float v = x[i];
+        y[i] = v > 0.f ? v : a * v;
+    }
+}
+
+__global__ void leaky_relu_grad_kernel(long n, float a, const float* __restrict__ x,
+                                       const float* __restrict__ gy, float* __restrict__ gx)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) gx[i] = x[i] > 0.f ? gy[i] : a * gy[i];
+}
+
+// y = alpha x + beta y, elementwise
+__global__ void scale_add_kernel(long n, float alpha, const float* __restrict__ x, float beta,
+                                 float* __restrict__ y)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = alpha * x[i] + beta * y[i];
+}
+
+// Adam step on n parameters (bias-corrected learning rate lr_t)
+__global__ void adam_kernel(long n, float lr_t, float b1, float b2, float eps,
+                            float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr_t * mi / (sqrtf(vi) + eps);
+}
+
+}  // namespace gpd
+
+using namespace gpd;
+
+static bool conv_dims_ok(int B, int Cin, int Cout, int D, int H, int W)
+{
+    return B > 0 && Cin > 0 && Cout > 0 && D > 0 && H > 0 && W > 0 && B <= 65535 &&
+           (long)D * H * W < (1L << 31);
+}
+
+extern "C" int gpd_conv3d_forward(int B, int Cin, int Cout, int D, int H, int W,
+                                  const float* d_x, const float* d_w, const float* d_bias,
+                                  float* d_y, int accumulate, void* stream)
+{
+    GPD_REQUIRE(conv_dims_ok(B, Cin, Cout, D, H, W), GPD_ERR_ARG, "bad conv sizes");
+    GPD_REQUIRE(d_x && d_w && d_y && d_x != d_y, GPD_ERR_ARG, "bad arrays");
+    ConvDims d = {B, Cin, Cout, D, H, W};
+    const long P = (long)D * H * W;
+    const int ocb = (Cout + kCOB - 1) / kCOB;
+    GPD_REQUIRE((long)B * ocb <= 65535, GPD_ERR_ARG, "batch too large");
+    hipLaunchKernelGGL(conv3d_kernel<0>, dim3((P + 255) / 256, B * ocb), dim3(256), 0,
+                       (hipStream_t)stream, d_x, d_w, d_bias, d_y, d, accumulate);
+    return last_error_to_status(hipGetLastError());
+}
+
+extern "C" int gpd_conv3d_backward_data(int B, int Cin, int Cout, int D, int H, int W,
+                                        const float* d_gy, const float* d_w, float* d_gx,
+                                        int accumulate, void* stream)
+{
+    GPD_REQUIRE(conv_dims_ok(B, Cin, Cout, D, H, W), GPD_ERR_ARG, "bad conv sizes");
+    GPD_REQUIRE(d_gy && d_w && d_gx && d_gy != d_gx, GPD_ERR_ARG, "bad arrays");
+    ConvDims d = {B, Cin, Cout, D, H, W};
+    const long P = (long)D * H * W;
+    const int icb = (Cin + kCOB - 1) / kCOB;
+    GPD_REQUIRE((long)B * icb <= 65535, GPD_ERR_ARG, "batch too large");
+    hipLaunchKernelGGL(conv3d_kernel<1>, dim3((P + 255) / 256, B * icb), dim3(256), 0,
+                       (hipStream_t)stream, d_gy, d_w, nullptr, d_gx, d, accumulate);
+    return last_error_to_status(hipGetLastError());
+}
+
+extern "C" int gpd_conv3d_backward_weight(int B, int Cin, int Cout, int D, int H, int W,
+                                          const float* d_x, const float* d_gy, float* d_gw,
+                                          float* d_gb, void* stream)
+{
+    GPD_REQUIRE(conv_dims_ok(B, Cin, Cout, D, H, W), GPD_ERR_ARG, "bad conv sizes");
+    GPD_REQUIRE(d_x && d_gy && d_gw, GPD_ERR_ARG, "bad arrays");
+    ConvDims d = {B, Cin, Cout, D, H, W};
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(conv3d_wgrad_kernel, dim3(Cout * Cin), dim3(256), 0, st, d_x, d_gy, d_gw,
+                       d);
+    int rc = last_error_to_status(hipGetLastError());
+    if (!rc && d_gb) {
+        hipLaunchKernelGGL(conv3d_bgrad_kernel, dim3(Cout), dim3(256), 0, st, d_gy, d_gb, d);
+        rc = last_error_to_status(hipGetLastError());
+    }
+    return rc;
+}
+
+extern "C" int gpd_leaky_relu(long n, float slope, const float* d_x, float* d_y, void* stream)
+{
+    GPD_REQUIRE(n >= 0 && (n == 0 || (d_x && d_y)), GPD_ERR_ARG, "bad arrays");
+    if (n == 0) return GPD_OK;
+    hipLaunchKernelGGL(leaky_relu_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, n, slope, d_x, d_y);
+    return last_error_to_status(hipGetLastError());
+}
+
+extern "C" int gpd_leaky_relu_backward(long n, float slope, const float* d_x, const float* d_gy,
+                                       float* d_gx, void* stream)
+{
+    GPD_REQUIRE(n >= 0 && (n == 0 || (d_x && d_gy && d_gx)), GPD_ERR_ARG, "bad arrays");
+    if (n == 0) return GPD_OK;
+    hipLaunchKernelGGL(leaky_relu_grad_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, n, slope, d_x, d_gy, d_gx);
+    return last_error_to_status(hipGetLastError());
+}
+
+extern "C" int gpd_adam_step(long n, float lr_t, float beta1, float beta2, float eps, float* d_w,
+                             const float* d_g, float* d_m, float* d_v, void* stream)
+{
+    GPD_REQUIRE(n >= 0 && (n == 0 || (d_w && d_g && d_m && d_v)), GPD_ERR_ARG, "bad arrays");
+    if (n == 0) return GPD_OK;
+    hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n,
+                       lr_t, beta1, beta2, eps, d_w, d_g, d_m, d_v);
+    return last_error_to_status(hipGetLastError());
+}
+
+extern "C" int gpd_scale_add(long n, float alpha, const float* d_x, float beta, float* d_y,
+                             void* stream)
+{
+    GPD_REQUIRE(n >= 0 && (n == 0 || (d_x && d_y)), GPD_ERR_ARG, "bad arrays");
+    if (n == 0) return GPD_OK;
+    hipLaunchKernelGGL(scale_add_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       n, alpha, d_x, beta, d_y);
+    return last_error_to_status(hipGetLastError());
+}

@@ -71,8 +71,7 @@ constexpr int kThreads = 256;
 
 template <typename T, int R> struct AxisCoefs {
     T c0;
-    T c[3][2 * R];   // per axis block: +1..+R then -1..-R
-    int order[3];    // axis of each block
+    T c[3][2 * R];   // per axis: +1..+R then -1..-R
 };
 
 // Workgroup barrier that orders LDS only: global loads issued before it
@@ -227,34 +226,26 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
                 if (!valid[k]) continue;
                 const int li = R + ty + 8 * k, lj = R + tx;
                 const T x = win[k][(ph + R) % W];
+                // centre, then axis 2, axis 1, axis 0, each +1..+R then
+                // -1..-R: the order of GPAW's Laplace stencils
                 T s = T(0);
                 s = madd<T>(s, x, cf.c0);
 #pragma unroll
-                for (int blk = 0; blk < 3; blk++) {
-                    const int ax = cf.order[blk];
-                    if (ax == 0) {
+                for (int j = 1; j <= R; j++) s = madd<T>(s, lds[buf][li][lj + j], cf.c[2][j - 1]);
 #pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, win[k][(ph + R + j) % W], cf.c[blk][j - 1]);
+                for (int j = 1; j <= R; j++)
+                    s = madd<T>(s, lds[buf][li][lj - j], cf.c[2][R + j - 1]);
 #pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, win[k][(ph + R - j) % W], cf.c[blk][R + j - 1]);
-                    } else if (ax == 1) {
+                for (int j = 1; j <= R; j++) s = madd<T>(s, lds[buf][li + j][lj], cf.c[1][j - 1]);
 #pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, lds[buf][li + j][lj], cf.c[blk][j - 1]);
+                for (int j = 1; j <= R; j++)
+                    s = madd<T>(s, lds[buf][li - j][lj], cf.c[1][R + j - 1]);
 #pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, lds[buf][li - j][lj], cf.c[blk][R + j - 1]);
-                    } else {
+                for (int j = 1; j <= R; j++)
+                    s = madd<T>(s, win[k][(ph + R + j) % W], cf.c[0][j - 1]);
 #pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, lds[buf][li][lj + j], cf.c[blk][j - 1]);
-#pragma unroll
-                        for (int j = 1; j <= R; j++)
-                            s = madd<T>(s, lds[buf][li][lj - j], cf.c[blk][R + j - 1]);
-                    }
-                }
+                for (int j = 1; j <= R; j++)
+                    s = madd<T>(s, win[k][(ph + R - j) % W], cf.c[0][R + j - 1]);
                 const long p = (long)z * plane + own_off[k];
                 T o;
                 if (EPI == EPI_PLAIN) {
@@ -329,8 +320,8 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
     AxisCoefs<T, R> cf;
     cf.c0 = (T)op->canon_c0;
     for (int blk = 0; blk < 3; blk++) {
-        cf.order[blk] = op->canon_order[blk];
-        for (int j = 0; j < 2 * R; j++) cf.c[blk][j] = (T)op->canon_c[blk][j];
+        const int ax = op->canon_order[blk];
+        for (int j = 0; j < 2 * R; j++) cf.c[ax][j] = (T)op->canon_c[blk][j];
     }
     return cf;
 }
@@ -363,18 +354,37 @@ static int launch_axis_R(const gpd_operator* op, long nb, const T* in, T* out,
     return last_error_to_status(hipGetLastError());
 }
 
+// Plain stencils: every range the axis kernel supports.
+template <typename T>
+static int launch_axis_plain(const gpd_operator* op, long nb, const T* in, T* out,
+                             hipStream_t st)
+{
+    const FdEpilogue<T> ep{};
+    switch (op->range) {
+    case 1: return launch_axis_R<T, 1, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case 2: return launch_axis_R<T, 2, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case 3: return launch_axis_R<T, 3, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case 4: return launch_axis_R<T, 4, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case 5: return launch_axis_R<T, 5, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    case 6: return launch_axis_R<T, 6, EPI_PLAIN, false>(op, nb, in, out, ep, st);
+    default: return GPD_ERR_UNSUPPORTED;
+    }
+}
+
+// Fused epilogues are instantiated for the ranges the Hamiltonian and the
+// preconditioner use (1..3: GPAW's default nn = 3 and the coarse levels'
+// nn = 1); kFusedRangeMissing sends the others through a temporary.
+constexpr int kFusedRangeMissing = 1;
+
 template <typename T, int EPI, bool LFC>
-static int launch_axis(const gpd_operator* op, long nb, const T* in, T* out,
-                       const FdEpilogue<T>& ep, hipStream_t st)
+static int launch_axis_fused(const gpd_operator* op, long nb, const T* in, T* out,
+                             const FdEpilogue<T>& ep, hipStream_t st)
 {
     switch (op->range) {
     case 1: return launch_axis_R<T, 1, EPI, LFC>(op, nb, in, out, ep, st);
     case 2: return launch_axis_R<T, 2, EPI, LFC>(op, nb, in, out, ep, st);
     case 3: return launch_axis_R<T, 3, EPI, LFC>(op, nb, in, out, ep, st);
-    case 4: return launch_axis_R<T, 4, EPI, LFC>(op, nb, in, out, ep, st);
-    case 5: return launch_axis_R<T, 5, EPI, LFC>(op, nb, in, out, ep, st);
-    case 6: return launch_axis_R<T, 6, EPI, LFC>(op, nb, in, out, ep, st);
-    default: return GPD_ERR_UNSUPPORTED;
+    default: return kFusedRangeMissing;
     }
 }
 
@@ -398,47 +408,77 @@ static int launch_generic(const gpd_operator* op, long nb, const T* in, T* out,
     return GPD_OK;
 }
 
+// Stencil into a stream-ordered temporary, then the epilogue: stencils
+// without the axis kernel (or its order), and ranges with no fused kernel.
+template <typename T>
+static int stencil_then_epilogue(const gpd_operator* op, bool axis, int epi, long nb,
+                                 const T* in, T* out, const FdEpilogue<T>& ep, hipStream_t st)
+{
+    const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+    T* s_buf = nullptr;
+    hipError_t e = hipMallocAsync((void**)&s_buf, sizeof(T) * nb * nG, st);
+    if (e != hipSuccess) {
+        set_last_error(hipGetErrorString(e));
+        return GPD_ERR_HIP;
+    }
+    int rc = axis ? launch_axis_plain<T>(op, nb, in, s_buf, st)
+                  : launch_generic<T>(op, nb, in, s_buf, st);
+    if (rc == GPD_OK) {
+        long n = nb * nG;
+        dim3 grid((n + 255) / 256);
+        if (epi == EPI_POT)
+            hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_POT>), grid, dim3(256), 0, st, s_buf,
+                               in, out, nG, nb, ep);
+        else if (epi == EPI_SUB)
+            hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB>), grid, dim3(256), 0, st, s_buf,
+                               in, out, nG, nb, ep);
+        else
+            hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB_AXPY>), grid, dim3(256), 0, st,
+                               s_buf, in, out, nG, nb, ep);
+        rc = last_error_to_status(hipGetLastError());
+    }
+    (void)hipFreeAsync(s_buf, st);
+    return rc;
+}
+
+// The axis kernel sums the blocks in the order axis 2, 1, 0 (GPAW's
+// Laplace stencils).  FP64 results are bit-exact only in the stencil's own
+// order, so FP64 stencils listed in another order take the generic kernel;
+// FP32 takes the axis kernel for any order.
+template <typename T>
+static bool axis_kernel_ok(const gpd_operator* op)
+{
+    if (!op->canonical) return false;
+    if (!std::is_same<T, double>::value) return true;
+    return op->canon_order[0] == 2 && op->canon_order[1] == 1 && op->canon_order[2] == 0;
+}
+
 template <typename T>
 int fd_apply_typed(const gpd_operator* op, int epi, long nb, const T* in, T* out,
                    const FdEpilogue<T>& ep, hipStream_t st)
 {
-    if (!op->canonical) {
-        if (epi == EPI_PLAIN) return launch_generic<T>(op, nb, in, out, st);
-        // generic stencil into a stream-ordered temporary, then the epilogue
-        const long nG = (long)op->n[0] * op->n[1] * op->n[2];
-        T* s_buf = nullptr;
-        hipError_t e = hipMallocAsync((void**)&s_buf, sizeof(T) * nb * nG, st);
-        if (e != hipSuccess) {
-            set_last_error(hipGetErrorString(e));
-            return GPD_ERR_HIP;
+    const bool axis = axis_kernel_ok<T>(op);
+    if (epi != EPI_PLAIN && epi != EPI_POT && epi != EPI_SUB && epi != EPI_SUB_AXPY)
+        return GPD_ERR_ARG;
+    if (epi == EPI_PLAIN)
+        return axis ? launch_axis_plain<T>(op, nb, in, out, st)
+                    : launch_generic<T>(op, nb, in, out, st);
+    int rc = kFusedRangeMissing;
+    if (axis) {
+        constexpr bool kF64 = std::is_same<T, double>::value;
+        if (epi == EPI_POT && ep.lfc_ptr) {
+            // the fused per-point projector sum is the exact FP64 path
+            if (kF64) rc = launch_axis_fused<T, EPI_POT, kF64>(op, nb, in, out, ep, st);
+        } else if (epi == EPI_POT) {
+            rc = launch_axis_fused<T, EPI_POT, false>(op, nb, in, out, ep, st);
+        } else if (epi == EPI_SUB) {
+            rc = launch_axis_fused<T, EPI_SUB, false>(op, nb, in, out, ep, st);
+        } else {
+            rc = launch_axis_fused<T, EPI_SUB_AXPY, false>(op, nb, in, out, ep, st);
         }
-        int rc = launch_generic<T>(op, nb, in, s_buf, st);
-        if (rc == GPD_OK) {
-            long n = nb * nG;
-            dim3 grid((n + 255) / 256);
-            if (epi == EPI_POT)
-                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_POT>), grid, dim3(256), 0, st, s_buf,
-                                   in, out, nG, nb, ep);
-            else if (epi == EPI_SUB)
-                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB>), grid, dim3(256), 0, st, s_buf,
-                                   in, out, nG, nb, ep);
-            else
-                hipLaunchKernelGGL((fd_epilogue_kernel<T, EPI_SUB_AXPY>), grid, dim3(256), 0, st,
-                                   s_buf, in, out, nG, nb, ep);
-            rc = last_error_to_status(hipGetLastError());
-        }
-        (void)hipFreeAsync(s_buf, st);
-        return rc;
     }
-    switch (epi) {
-    case EPI_PLAIN: return launch_axis<T, EPI_PLAIN, false>(op, nb, in, out, ep, st);
-    case EPI_POT:
-        if (ep.lfc_ptr) return launch_axis<T, EPI_POT, true>(op, nb, in, out, ep, st);
-        return launch_axis<T, EPI_POT, false>(op, nb, in, out, ep, st);
-    case EPI_SUB: return launch_axis<T, EPI_SUB, false>(op, nb, in, out, ep, st);
-    case EPI_SUB_AXPY: return launch_axis<T, EPI_SUB_AXPY, false>(op, nb, in, out, ep, st);
-    default: return GPD_ERR_ARG;
-    }
+    if (rc != kFusedRangeMissing) return rc;
+    return stencil_then_epilogue<T>(op, axis, epi, nb, in, out, ep, st);
 }
 
 template int fd_apply_typed<double>(const gpd_operator*, int, long, const double*, double*,

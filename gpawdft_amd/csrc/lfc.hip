@@ -332,29 +332,138 @@ template <typename V>
 static hipError_t upload(V** dst, const V* src, size_t n)
 {
     *dst = nullptr;
-    if (n == 0) n = 1;   // keep a valid pointer for empty tables
-    hipError_t e = hipMalloc((void**)dst, sizeof(V) * n);
+    // keep a valid pointer for empty tables
+    hipError_t e = hipMalloc((void**)dst, sizeof(V) * (n ? n : 1));
     if (e != hipSuccess) return e;
-    if (src) return hipMemcpy(*dst, src, sizeof(V) * n, hipMemcpyHostToDevice);
-    return hipMemset(*dst, 0, sizeof(V) * n);
+    if (src && n) return hipMemcpy(*dst, src, sizeof(V) * n, hipMemcpyHostToDevice);
+    return hipMemset(*dst, 0, sizeof(V) * (n ? n : 1));
 }
 
 int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st)
 {
     constexpr int NB = 8;
     const long nbblk = (nx + NB - 1) / NB;
-    for (int col = 0; col < l->ncolours; col++) {
-        const int i0 = l->colour_items[col], i1 = l->colour_items[col + 1];
+    const LfcGroups& gs = l->ga;
+    for (int col = 0; col < gs.ncolours; col++) {
+        const int i0 = gs.colour_items[col], i1 = gs.colour_items[col + 1];
         if (i1 == i0) continue;
         const long nwork = (long)(i1 - i0) * nbblk;
         hipLaunchKernelGGL(lfc_add_grp_kernel<NB>, dim3(xcd_grid_size(nwork)), dim3(256), 0, st, c,
-                           a, l->nG, l->Mmax, nx, l->nfmax, i1 - i0, l->item_g + i0,
-                           l->item_j0 + i0, l->grp_off, l->grp_nf, l->grp_fM, l->grp_G,
-                           l->grp_A32);
+                           a, l->nG, l->Mmax, nx, gs.nfmax, i1 - i0, gs.item_g + i0,
+                           gs.item_j0 + i0, gs.grp_off, gs.grp_nf, gs.grp_fM, gs.grp_G,
+                           gs.grp_A32);
         int rc = last_error_to_status(hipGetLastError());
         if (rc) return rc;
     }
     return GPD_OK;
+}
+
+
+// Groups of keys with identical point lists (<= kNF functions each, keys
+// taken in M order), their FP32 A tables and a greedy colouring into sets
+// of groups with disjoint points; uploads everything into gs.
+static hipError_t build_groups(const std::vector<int>& key_M, const std::vector<int>& key_nm,
+                               const std::vector<std::vector<int>>& key_G,
+                               const std::vector<std::vector<int>>& key_row, const double* A,
+                               long nG, LfcGroups& gs)
+{
+    const int nkeys = (int)key_M.size();
+    std::vector<int> order(nkeys);
+    for (int k = 0; k < nkeys; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return key_M[x] < key_M[y]; });
+    std::vector<std::vector<int>> groups;
+    std::vector<int> group_nf;
+    for (int k : order) {
+        int found = -1;
+        for (int g = (int)groups.size() - 1; g >= 0 && g >= (int)groups.size() - 8; g--) {
+            int k0 = groups[g][0];
+            if (group_nf[g] + key_nm[k] <= kNF && key_G[k0] == key_G[k]) {
+                found = g;
+                break;
+            }
+        }
+        if (found < 0) {
+            groups.push_back({k});
+            group_nf.push_back(key_nm[k]);
+        } else {
+            groups[found].push_back(k);
+            group_nf[found] += key_nm[k];
+        }
+    }
+    gs.ngroups = (int)groups.size();
+    int nfmax = 1;
+    for (int nf : group_nf) nfmax = std::max(nfmax, nf);
+    gs.nfmax = nfmax;
+    std::vector<long> grp_off(gs.ngroups + 1, 0);
+    for (int g = 0; g < gs.ngroups; g++)
+        grp_off[g + 1] = grp_off[g] + (long)key_G[groups[g][0]].size();
+    gs.npts_total = grp_off[gs.ngroups];
+    std::vector<int> gG(gs.npts_total), gfM((size_t)gs.ngroups * nfmax, 0);
+    std::vector<float> gA((size_t)gs.npts_total * nfmax, 0.f);
+    for (int g = 0; g < gs.ngroups; g++) {
+        const std::vector<int>& pts = key_G[groups[g][0]];
+        std::copy(pts.begin(), pts.end(), gG.begin() + grp_off[g]);
+        int f = 0;
+        for (int k : groups[g]) {
+            for (int m = 0; m < key_nm[k]; m++, f++) {
+                gfM[(size_t)g * nfmax + f] = key_M[k] + m;
+                for (size_t j = 0; j < pts.size(); j++)
+                    gA[(grp_off[g] + j) * nfmax + f] = (float)A[key_row[k][j] + m];
+            }
+        }
+    }
+
+    std::vector<int> colour_of(gs.ngroups, -1);
+    std::vector<std::vector<char>> used;
+    for (int g = 0; g < gs.ngroups; g++) {
+        const std::vector<int>& pts = key_G[groups[g][0]];
+        int col = 0;
+        for (;; col++) {
+            if (col == (int)used.size()) used.emplace_back(nG, 0);
+            bool clash = false;
+            for (int G : pts)
+                if (used[col][G]) {
+                    clash = true;
+                    break;
+                }
+            if (!clash) break;
+        }
+        for (int G : pts) used[col][G] = 1;
+        colour_of[g] = col;
+    }
+    gs.ncolours = (int)used.size();
+    std::vector<int> item_g;
+    std::vector<long> item_j0;
+    gs.colour_items.assign(gs.ncolours + 1, 0);
+    for (int col = 0; col < gs.ncolours; col++) {
+        gs.colour_items[col] = (int)item_g.size();
+        for (int g = 0; g < gs.ngroups; g++)
+            if (colour_of[g] == col)
+                for (long j = grp_off[g]; j < grp_off[g + 1]; j += 256) {
+                    item_g.push_back(g);
+                    item_j0.push_back(j);
+                }
+    }
+    gs.colour_items[gs.ncolours] = (int)item_g.size();
+
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = upload(&gs.grp_off, grp_off.data(), grp_off.size());
+    if (e == hipSuccess) e = upload(&gs.grp_nf, group_nf.data(), group_nf.size());
+    if (e == hipSuccess) e = upload(&gs.grp_fM, gfM.data(), gfM.size());
+    if (e == hipSuccess) e = upload(&gs.grp_G, gG.data(), gG.size());
+    if (e == hipSuccess) e = upload(&gs.grp_A32, gA.data(), gA.size());
+    if (e == hipSuccess) e = upload(&gs.item_g, item_g.data(), item_g.size());
+    if (e == hipSuccess) e = upload(&gs.item_j0, item_j0.data(), item_j0.size());
+    return e;
+}
+
+static void free_groups(LfcGroups& gs)
+{
+    void* ptrs[] = {gs.grp_off, gs.grp_nf, gs.grp_fM, gs.grp_G, gs.grp_A32, gs.item_g, gs.item_j0};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    gs = LfcGroups();
 }
 
 }  // namespace gpd
@@ -365,10 +474,11 @@ extern "C" int gpd_lfc_destroy(gpd_lfc* l)
 {
     if (!l) return GPD_OK;
     void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
-                    l->key_off, l->key_G, l->key_row, l->grp_off, l->grp_nf, l->grp_fM,
-                    l->grp_G, l->grp_A32, l->item_g, l->item_j0};
+                    l->key_off, l->key_G, l->key_row};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    free_groups(l->gi);
+    if (!l->add_shares_groups) free_groups(l->ga);
     delete l;
     return GPD_OK;
 }
@@ -414,88 +524,42 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
         kR.insert(kR.end(), h.key_row[k].begin(), h.key_row[k].end());
     }
 
-    // groups: keys (in M order) with identical point lists, <= 16 functions
-    std::vector<int> order(l->nkeys);
-    for (int k = 0; k < l->nkeys; k++) order[k] = k;
-    std::sort(order.begin(), order.end(), [&](int x, int y) { return h.key_M[x] < h.key_M[y]; });
-    std::vector<std::vector<int>> groups;
-    std::vector<int> group_nf;
-    for (int k : order) {
-        int found = -1;
-        for (int g = (int)groups.size() - 1; g >= 0 && g >= (int)groups.size() - 8; g--) {
-            int k0 = groups[g][0];
-            if (group_nf[g] + h.key_nm[k] <= kNF && h.key_G[k0] == h.key_G[k]) {
-                found = g;
-                break;
+    hipError_t e = build_groups(h.key_M, h.key_nm, h.key_G, h.key_row, A, nG, l->gi);
+    // sub-keys for the grouped add: the j-th visit of a point in a key's
+    // list goes to sub-key j, so no sub-key repeats a point
+    std::vector<int> sk_M, sk_nm;
+    std::vector<std::vector<int>> sk_G, sk_row;
+    bool repeats = false;
+    {
+        std::vector<int> seen(nG, 0);
+        for (int k = 0; k < l->nkeys; k++) {
+            std::vector<int> sub_of;   // occurrence -> sub-key index
+            for (size_t j = 0; j < h.key_G[k].size(); j++) {
+                const int G = h.key_G[k][j];
+                const int occ = seen[G]++;
+                if (occ > 0) repeats = true;
+                if (occ == (int)sub_of.size()) {
+                    sub_of.push_back((int)sk_M.size());
+                    sk_M.push_back(h.key_M[k]);
+                    sk_nm.push_back(h.key_nm[k]);
+                    sk_G.emplace_back();
+                    sk_row.emplace_back();
+                }
+                sk_G[sub_of[occ]].push_back(G);
+                sk_row[sub_of[occ]].push_back(h.key_row[k][j]);
             }
-        }
-        if (found < 0) {
-            groups.push_back({k});
-            group_nf.push_back(h.key_nm[k]);
-        } else {
-            groups[found].push_back(k);
-            group_nf[found] += h.key_nm[k];
+            for (int G : h.key_G[k]) seen[G] = 0;
         }
     }
-    l->ngroups = (int)groups.size();
-    int nfmax = 1;
-    for (int nf : group_nf) nfmax = std::max(nfmax, nf);
-    l->nfmax = nfmax;
-    std::vector<long> grp_off(l->ngroups + 1, 0);
-    for (int g = 0; g < l->ngroups; g++)
-        grp_off[g + 1] = grp_off[g] + (long)h.key_G[groups[g][0]].size();
-    l->npts_total = grp_off[l->ngroups];
-    std::vector<int> gG(l->npts_total), gfM((size_t)l->ngroups * nfmax, 0);
-    std::vector<float> gA((size_t)l->npts_total * nfmax, 0.f);
-    for (int g = 0; g < l->ngroups; g++) {
-        const std::vector<int>& pts = h.key_G[groups[g][0]];
-        std::copy(pts.begin(), pts.end(), gG.begin() + grp_off[g]);
-        int f = 0;
-        for (int k : groups[g]) {
-            for (int m = 0; m < h.key_nm[k]; m++, f++) {
-                gfM[(size_t)g * nfmax + f] = h.key_M[k] + m;
-                for (size_t j = 0; j < pts.size(); j++)
-                    gA[(grp_off[g] + j) * nfmax + f] = (float)A[h.key_row[k][j] + m];
-            }
-        }
+    if (!repeats) {
+        l->ga = l->gi;
+        l->add_shares_groups = true;
+    } else {
+        l->add_shares_groups = false;
+        if (e == hipSuccess) e = build_groups(sk_M, sk_nm, sk_G, sk_row, A, nG, l->ga);
     }
     std::vector<float> A32(A, A + l->nvalues);
 
-    // colour the groups: groups of one colour cover disjoint points
-    std::vector<int> colour_of(l->ngroups, -1);
-    std::vector<std::vector<char>> used;
-    for (int g = 0; g < l->ngroups; g++) {
-        const std::vector<int>& pts = h.key_G[groups[g][0]];
-        int col = 0;
-        for (;; col++) {
-            if (col == (int)used.size()) used.emplace_back(nG, 0);
-            bool clash = false;
-            for (int G : pts)
-                if (used[col][G]) {
-                    clash = true;
-                    break;
-                }
-            if (!clash) break;
-        }
-        for (int G : pts) used[col][G] = 1;
-        colour_of[g] = col;
-    }
-    l->ncolours = (int)used.size();
-    std::vector<int> item_g;
-    std::vector<long> item_j0;
-    l->colour_items.assign(l->ncolours + 1, 0);
-    for (int col = 0; col < l->ncolours; col++) {
-        l->colour_items[col] = (int)item_g.size();
-        for (int g = 0; g < l->ngroups; g++)
-            if (colour_of[g] == col)
-                for (long j = grp_off[g]; j < grp_off[g + 1]; j += 256) {
-                    item_g.push_back(g);
-                    item_j0.push_back(j);
-                }
-    }
-    l->colour_items[l->ncolours] = (int)item_g.size();
-
-    hipError_t e = hipSuccess;
 #define UP(dst, src, n) \
     if (e == hipSuccess) e = upload(&l->dst, src, n)
     UP(A64, A, l->nvalues);
@@ -509,13 +573,6 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     UP(key_off, key_off.data(), key_off.size());
     UP(key_G, kG.data(), kG.size());
     UP(key_row, kR.data(), kR.size());
-    UP(grp_off, grp_off.data(), grp_off.size());
-    UP(grp_nf, group_nf.data(), group_nf.size());
-    UP(grp_fM, gfM.data(), gfM.size());
-    UP(grp_G, gG.data(), gG.size());
-    UP(grp_A32, gA.data(), gA.size());
-    UP(item_g, item_g.data(), item_g.size());
-    UP(item_j0, item_j0.data(), item_j0.size());
 #undef UP
     if (e != hipSuccess) {
         set_last_error(hipGetErrorString(e));
@@ -543,11 +600,12 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
     }
     if (dtype == GPD_F32) {
         constexpr int NBB = 4;
-        long nwork = ((nx + NBB - 1) / NBB) * l->ngroups;
+        const LfcGroups& gs = l->gi;
+        long nwork = ((nx + NBB - 1) / NBB) * gs.ngroups;
         hipLaunchKernelGGL((lfc_integrate_grp_kernel<float, NBB>), dim3(xcd_grid_size(nwork)),
                            dim3(256), 0, st, (const float*)d_a_xG, (float*)d_c_xM, l->nG, l->Mmax,
-                           nx, (float)l->dv, l->ngroups, l->nfmax, l->grp_off, l->grp_nf,
-                           l->grp_fM, l->grp_G, l->grp_A32);
+                           nx, (float)l->dv, gs.ngroups, gs.nfmax, gs.grp_off, gs.grp_nf,
+                           gs.grp_fM, gs.grp_G, gs.grp_A32);
         return last_error_to_status(hipGetLastError());
     }
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");

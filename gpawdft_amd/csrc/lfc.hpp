@@ -16,6 +16,30 @@
 // has a point list (G, row offset) in the order the sweep visits them for
 // that key.  Keys with identical point lists form a "group" so one pass
 // over the points serves all of their functions.
+//
+// FP32 fast paths use such groups.  A sphere larger than the cell overlaps
+// its own periodic image, so a key's point list can hold one point twice;
+// the grouped add writes each point of a group from one thread, so for add
+// such keys are split into sub-keys by occurrence (first visits of each
+// point, second visits, ...), which have distinct points and overlap each
+// other, so colouring puts them in different launches.
+struct LfcGroups {
+    int ngroups = 0;
+    int nfmax = 0;             // max functions in a group
+    long npts_total = 0;       // sum of group point-list lengths
+    long* grp_off = nullptr;   // ngroups + 1 into grp_G and rows of grp_A
+    int* grp_nf = nullptr;     // functions per group
+    int* grp_fM = nullptr;     // ngroups * nfmax: M index of each function
+    int* grp_G = nullptr;      // npts_total
+    float* grp_A32 = nullptr;  // npts_total * nfmax (row-major, zero padded)
+    // colours of groups with disjoint points; work items are (group, first
+    // point) chunks of 256 points, listed colour by colour
+    int ncolours = 0;
+    std::vector<int> colour_items;   // host: ncolours + 1 offsets into items
+    int* item_g = nullptr;
+    long* item_j0 = nullptr;
+};
+
 struct gpd_lfc {
     long nG = 0;
     int Mmax = 0;
@@ -24,9 +48,6 @@ struct gpd_lfc {
     long ncov = 0;             // covered points
     long nent = 0;             // point entries
     int nkeys = 0;
-    int ngroups = 0;
-    int nfmax = 0;             // max functions in a group
-    long npts_total = 0;       // sum of group point-list lengths
     // device arrays
     double* A64 = nullptr;     // nvalues
     float* A32 = nullptr;      // nvalues
@@ -40,18 +61,9 @@ struct gpd_lfc {
     long* key_off = nullptr;   // nkeys + 1 into key_G / key_row
     int* key_G = nullptr;
     int* key_row = nullptr;
-    // integrate, grouped (FP32)
-    long* grp_off = nullptr;   // ngroups + 1 into grp_G and rows of grp_A
-    int* grp_nf = nullptr;     // functions per group
-    int* grp_fM = nullptr;     // ngroups * nfmax: M index of each function
-    int* grp_G = nullptr;      // npts_total
-    float* grp_A32 = nullptr;  // npts_total * nfmax (row-major, zero padded)
-    // grouped add: colours of groups with disjoint points; work items are
-    // (group, first point) chunks of 256 points, listed colour by colour
-    int ncolours = 0;
-    std::vector<int> colour_items;   // host: ncolours + 1 offsets into items
-    int* item_g = nullptr;
-    long* item_j0 = nullptr;
+    LfcGroups gi;              // grouped integrate (FP32), from the keys
+    LfcGroups ga;              // grouped add (FP32), from the sub-keys
+    bool add_shares_groups = true;   // no repeated points: ga aliases gi
 };
 
 namespace gpd {

@@ -4,11 +4,13 @@
 // 251-313, 417-466 via gpaw/grid_descriptor.py:263-323 and c/blas.c:
 // dsyrk/dsyr2k/dgemm):
 //  * inner products T = X Y^T over the grid (S: X = Y = psi; H: X = psi,
-//    Y = H psi, symmetrised).  FP32 inputs: each value is split into two
-//    bfloat16 parts x = hi + lo and the four products hi.hi, hi.lo,
-//    lo.hi, lo.lo run on v_mfma_f32_32x32x8_bf16 with FP32 accumulation
-//    inside a K-chunk; the chunks' partial matrices are summed in FP64 by a
-//    second kernel (deterministic order).  FP64 inputs: v_mfma_f64_16x16x4.
+//    Y = H psi, symmetrised).  FP32 inputs: each value is split into three
+//    bfloat16 parts x = hi + mid + lo and the six products whose size is
+//    above 2^-24 |x y| run on v_mfma_f32_32x32x8_bf16; the FP32 accumulator is
+//    flushed into FP64 registers every 64 values of k, and the K-chunks'
+//    partial matrices are summed in FP64 by a second kernel (fixed order).
+//    FP64 inputs: v_mfma_f64_16x16x4 (its result row is 4 v + lane / 16,
+//    column lane % 16, for accumulator register v).
 //  * rotations psi' = C psi (C: FP64 N x N): FP32 psi on
 //    v_mfma_f32_32x32x2_f32 with the accumulator flushed into FP64
 //    registers every 32 terms; FP64 psi on v_mfma_f64_16x16x4.
@@ -37,23 +39,27 @@ __device__ __forceinline__ float bf16_value(unsigned short b)
     return __uint_as_float(((unsigned int)b) << 16);
 }
 
-// x = hi + lo with hi, lo bfloat16
-__device__ __forceinline__ void split2(float x, short& hi, short& lo)
+// x = hi + mid + lo with bfloat16 parts (the remainder after three
+// parts is below 2^-24 |x|; all subtractions are exact in FP32)
+__device__ __forceinline__ void split3(float x, short& hi, short& mid, short& lo)
 {
-    unsigned short h = bf16_bits(x);
-    float r = x - bf16_value(h);
+    const unsigned short h = bf16_bits(x);
+    const float r1 = x - bf16_value(h);
+    const unsigned short m = bf16_bits(r1);
+    const float r2 = r1 - bf16_value(m);
     hi = (short)h;
-    lo = (short)bf16_bits(r);
+    mid = (short)m;
+    lo = (short)bf16_bits(r2);
 }
 
 // ------------------------------------------------------------------ //
 // split-bf16 inner products: partial[s][i][j] = sum_{k in chunk s} X[i,k] Y[j,k]
 // Workgroup: 64 x 64 output tile, four waves of 32 x 32.
 
-__global__ __launch_bounds__(256) void inner_bf16x2_kernel(const float* __restrict__ X,
+__global__ __launch_bounds__(256) void inner_bf16x3_kernel(const float* __restrict__ X,
                                                           const float* __restrict__ Y, int N,
                                                           long K, long kchunk, int nsplit,
-                                                          float* __restrict__ partial)
+                                                          double* __restrict__ partial)
 {
     const int tiles = (N + 63) / 64;
     const long nwork = (long)tiles * tiles * nsplit;
@@ -73,35 +79,54 @@ __global__ __launch_bounds__(256) void inner_bf16x2_kernel(const float* __restri
     const long k_begin = (long)s * kchunk;
     const long k_end = std::min(K, k_begin + kchunk);
 
+    // FP32 MFMA accumulation over 64 values of k at a time, then FP64
     f32x16 acc = {};
+    double acc64[16];
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc64[v] = 0.0;
+    int steps = 0;
     for (long k0 = k_begin; k0 < k_end; k0 += 8) {
-        bf16x4 ah, al, bh, bl;
+        bf16x4 ah, am, al, bh, bm, bl;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const long k = k0 + kq + q;
             const bool in = k < k_end;
             const float a = (va && in) ? xa[k] : 0.f;
             const float b = (vb && in) ? yb[k] : 0.f;
-            short h, l;
-            split2(a, h, l);
+            short h, m, l;
+            split3(a, h, m, l);
             ah[q] = h;
+            am[q] = m;
             al[q] = l;
-            split2(b, h, l);
+            split3(b, h, m, l);
             bh[q] = h;
+            bm[q] = m;
             bl[q] = l;
         }
+        // the six products down to 2^-16 relative of x y; the dropped
+        // ones (mid.lo, lo.mid, lo.lo) are below 2^-24
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bm, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(am, bh, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(ah, bl, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bh, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(al, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(am, bm, acc, 0, 0, 0);
+        if (++steps == 8) {
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                acc64[v] += (double)acc[v];
+                acc[v] = 0.f;
+            }
+            steps = 0;
+        }
     }
     // D layout: column j0 + lane % 32, rows 8 (v / 4) + 4 (lane / 32) + v % 4
-    float* P = partial + (long)s * N * N;
+    double* P = partial + (long)s * N * N;
     const int jo = j0 + lane % 32;
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int io = i0 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-        if (io < N && jo < N) P[(long)io * N + jo] = acc[v];
+        if (io < N && jo < N) P[(long)io * N + jo] = acc64[v] + (double)acc[v];
     }
 }
 
@@ -159,7 +184,7 @@ __global__ __launch_bounds__(256) void inner_f64_kernel(const double* __restrict
             const int jo = j0 + 16 * q + r16;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int io = i0 + 16 * p + 4 * kk + v;
+                const int io = i0 + 16 * p + 4 * v + kk;   // f64 D: row 4v + lane/16
                 if (io < N && jo < N) P[(long)io * N + jo] = acc[p][q][v];
             }
         }
@@ -271,7 +296,7 @@ __global__ __launch_bounds__(256) void rotate_f64_kernel(const double* __restric
         const long ko = k0 + 16 * q + r16;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int no = n0 + 4 * kk + v;
+            const int no = n0 + 4 * v + kk;
             if (no < N && ko < K) Y[(long)no * K + ko] = acc[q][v];
         }
     }
@@ -322,8 +347,7 @@ extern "C" int gpd_inner_products(int dtype, int N, long K, const void* d_X, con
     GPD_REQUIRE(d_X && d_Y && d_out, GPD_ERR_ARG, "null array");
     hipStream_t st = (hipStream_t)stream;
     const int nsplit = pick_nsplit(N, K);
-    const size_t esize = dtype == GPD_F64 ? sizeof(double) : sizeof(float);
-    const size_t need = (size_t)nsplit * N * N * esize;
+    const size_t need = (size_t)nsplit * N * N * sizeof(double);
     void* work = d_work;
     bool own = false;
     if (!work || (size_t)work_bytes < need) {
@@ -336,14 +360,14 @@ extern "C" int gpd_inner_products(int dtype, int N, long K, const void* d_X, con
     const long nwork = (long)tiles * tiles * nsplit;
     int rc = GPD_OK;
     if (dtype == GPD_F32) {
-        hipLaunchKernelGGL(inner_bf16x2_kernel, dim3(xcd_grid_size(nwork)), dim3(256), 0, st,
+        hipLaunchKernelGGL(inner_bf16x3_kernel, dim3(xcd_grid_size(nwork)), dim3(256), 0, st,
                            (const float*)d_X, (const float*)d_Y, N, K, kchunk, nsplit,
-                           (float*)work);
+                           (double*)work);
         rc = last_error_to_status(hipGetLastError());
         if (!rc) {
             long n = (long)N * N;
-            hipLaunchKernelGGL(reduce_partials_kernel<float>, dim3((n + 255) / 256), dim3(256),
-                               0, st, (const float*)work, N, nsplit, alpha, hermitian, d_out);
+            hipLaunchKernelGGL(reduce_partials_kernel<double>, dim3((n + 255) / 256), dim3(256),
+                               0, st, (const double*)work, N, nsplit, alpha, hermitian, d_out);
             rc = last_error_to_status(hipGetLastError());
         }
     } else if (dtype == GPD_F64) {

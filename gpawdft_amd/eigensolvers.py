@@ -84,10 +84,17 @@ class WaveFunctions:
     def zeros_P(self, n):
         return torch.zeros((n, self.pt.Mmax), dtype=self.dtype, device='cuda')
 
-    def apply_hamiltonian(self, ham, psit_xG, Htpsit_xG, P_xM=None):
-        """H psit (kinetic + local + projectors); returns the projections."""
-        return ham.apply_fused(psit_xG, Htpsit_xG, self.kin, self.pt,
-                               P_xM=P_xM)
+    def apply_pseudo_hamiltonian(self, ham, psit_xG, Htpsit_xG):
+        """(T + v) psit: kinetic stencil with the local potential in its
+        epilogue (gpaw/wavefunctions/fd.py:77-83); the projector part is
+        added by the callers."""
+        ham.apply_fused(psit_xG, Htpsit_xG, self.kin, None)
+
+    def project(self, psit_xG, P_xM):
+        """P_xM = <p|psit> (LFC.integrate)."""
+        P_xM.zero_()
+        self.pt.integrate_xM(psit_xG, P_xM)
+        return P_xM
 
 
 class KPoint:
@@ -167,7 +174,7 @@ class Eigensolver:
         Htpsit = self.Htpsit_nG
 
         def H(psit_xG):
-            wfs.apply_hamiltonian(ham, psit_xG, Htpsit)
+            wfs.apply_pseudo_hamiltonian(ham, psit_xG, Htpsit)
             return Htpsit
 
         dH = {a: unpack(ham.dH_asp[a][kpt.s]) for a in wfs.pt.atom_indices}
@@ -234,7 +241,8 @@ class RMMDIIS(Eigensolver):
                 R_diis.view(B, niter, *R_x.shape[1:])[:Bx, 0] = R_x
             dpsit_x = pc(R_x, kpt)
             P_x = P_d[:Bx]
-            wfs.apply_hamiltonian(ham, dpsit_x, dR_x, P_x)
+            wfs.apply_pseudo_hamiltonian(ham, dpsit_x, dR_x)
+            wfs.project(dpsit_x, P_x)
             self.calculate_residuals(kpt, wfs, ham, dpsit_x, P_x, eps_x,
                                      dR_x)
             RdR = dots(dR_x, R_x, dv)
@@ -262,7 +270,8 @@ class RMMDIIS(Eigensolver):
                 if nit < niter - 1:
                     dpsit_x = pc(R_x, kpt)
                     axpy(lam_x, dpsit_x, psit_x)
-                    wfs.apply_hamiltonian(ham, psit_x, R_x, P_x)
+                    wfs.apply_pseudo_hamiltonian(ham, psit_x, R_x)
+                    wfs.project(psit_x, P_x)
                     self.calculate_residuals(kpt, wfs, ham, psit_x, P_x,
                                              eps_x, R_x)
             dpsit_x = pc(R_x, kpt)
@@ -278,9 +287,10 @@ class RMMDIIS(Eigensolver):
         Bx = pd.shape[0]
         m = nit + 1
         R_bnn = np.empty((Bx, m, m))
+        rcols = [rd[:, i].contiguous() for i in range(m)]
         for i in range(m):
             for j in range(i, m):
-                v = dots(rd[:, i], rd[:, j], dv)
+                v = dots(rcols[i], rcols[j], dv)
                 R_bnn[:, i, j] = v
                 R_bnn[:, j, i] = v
         alpha = np.empty((Bx, m))
@@ -295,4 +305,4 @@ class RMMDIIS(Eigensolver):
         R_x.zero_()
         for i in range(m):
             axpy(alpha[:, i], pd[:, i].contiguous(), psit_x)
-            axpy(alpha[:, i], rd[:, i].contiguous(), R_x)
+            axpy(alpha[:, i], rcols[i], R_x)

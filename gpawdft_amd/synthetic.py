@@ -51,13 +51,25 @@ def gaussian_wells(gd, natoms, seed):
     return v, spos_ac
 
 
-def projector_splines(lmax=2, rc_ang=1.2, rmax_ang=2.5, npts=201):
+def projector_splines(lmax=2, rc_ang=1.2, rmax_ang=2.5, npts=201,
+                      normalize=False):
+    """Gaussian radial parts f_l(r) = exp(-r^2/rc^2); the grid function is
+    f_l(r) r^l Y_lm.  With normalize=True each f_l is scaled so that
+    int |f_l r^l Y_lm|^2 dV = 1, as PAW projectors of unit norm."""
     from gpawdft_amd.spline import Spline
     rc = rc_ang / Bohr
     rmax = rmax_ang / Bohr
     r = np.linspace(0.0, rmax, npts)
     f = np.exp(-r**2 / rc**2)
-    return [Spline(l, rmax, f) for l in range(lmax + 1)]
+    out = []
+    for l in range(lmax + 1):
+        fl = f
+        if normalize:
+            g = (f * r**(l + 1))**2
+            norm = np.sum(0.5 * (g[1:] + g[:-1]) * np.diff(r))
+            fl = f / np.sqrt(norm)
+        out.append(Spline(l, rmax, fl))
+    return out
 
 
 def random_dH(natoms, ni=9, seed=0, scale=0.1):
@@ -109,12 +121,13 @@ def smooth_bands_device(nb, n_c, seed, sigma=1.0, dtype=None, device='cuda'):
 class System:
     """A periodic grid with potential, projectors and dH (one spin)."""
 
-    def __init__(self, N, h_ang, natoms, seed=0, nn=3, lmax=2):
+    def __init__(self, N, h_ang, natoms, seed=0, nn=3, lmax=2,
+                 unit_projectors=True):
         self.gd = make_grid(N, h_ang)
         self.vt_G, self.well_spos = gaussian_wells(self.gd, natoms, seed)
         rng = np.random.default_rng(seed + 1)
         self.spos_ac = rng.uniform(0.0, 1.0, (natoms, 3))
-        self.splines = projector_splines(lmax)
+        self.splines = projector_splines(lmax, normalize=unit_projectors)
         ni = sum(2 * l + 1 for l in range(lmax + 1))
         self.dH_asp = random_dH(natoms, ni, seed + 2)
         self.natoms = natoms

@@ -191,9 +191,9 @@ int gpd_precond_destroy(gpd_precond* pc);
  * N (full N x N, FP64, row-major); hermitian != 0 gives
  * 0.5 alpha (T + T^T) -- the rk / r2k calls of GridDescriptor.integrate
  * (gpaw/grid_descriptor.py:300-306, c/blas.c:290-353), both triangles
- * filled.  X, Y: (N, K) of dtype.  FP32 runs split-bfloat16 MFMA (hi+lo
- * per operand, four products, FP32 accumulation per K-chunk, FP64 sum of
- * the chunks); FP64 runs FP64 MFMA.  d_work (may be NULL) is device
+ * filled.  X, Y: (N, K) of dtype.  FP32 runs split-bfloat16 MFMA
+ * (hi+mid+lo per operand, six products, FP32 accumulation over 64 values
+ * of k, then FP64); FP64 runs FP64 MFMA.  d_work (may be NULL) is device
  * scratch for the per-chunk partials; if too small, the call allocates.  */
 int gpd_inner_products(int dtype, int N, long K, const void* d_X,
                        const void* d_Y, double alpha, int hermitian,
@@ -243,6 +243,35 @@ int gpd_axpy_batched(int dtype, long nb, long nG, const double* d_coef,
                      const void* d_x, void* d_y, void* stream);
 int gpd_dots(int dtype, long nb, long nG, double alpha, const void* d_a,
              const void* d_b, double* d_out, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* Learned preconditioner kernels (no counterpart in the reference: the
+ * network stands beside the multigrid Preconditioner).  FP32 tensors
+ * (B, C, D, H, W), C-contiguous; periodic 3x3x3 convolutions with weights
+ * (Cout, Cin, 27) and tap t = 9 dz + 3 dy + dx (offsets -1..1).
+ * accumulate != 0 adds into y (or gx) instead of overwriting.           */
+int gpd_conv3d_forward(int B, int Cin, int Cout, int D, int H, int W,
+                       const float* d_x, const float* d_w,
+                       const float* d_bias, float* d_y, int accumulate,
+                       void* stream);
+int gpd_conv3d_backward_data(int B, int Cin, int Cout, int D, int H, int W,
+                             const float* d_gy, const float* d_w,
+                             float* d_gx, int accumulate, void* stream);
+/* gw (Cout, Cin, 27) and gb (Cout, may be NULL) are overwritten.        */
+int gpd_conv3d_backward_weight(int B, int Cin, int Cout, int D, int H, int W,
+                               const float* d_x, const float* d_gy,
+                               float* d_gw, float* d_gb, void* stream);
+int gpd_leaky_relu(long n, float slope, const float* d_x, float* d_y,
+                   void* stream);
+int gpd_leaky_relu_backward(long n, float slope, const float* d_x,
+                            const float* d_gy, float* d_gx, void* stream);
+/* y = alpha x + beta y */
+int gpd_scale_add(long n, float alpha, const float* d_x, float beta,
+                  float* d_y, void* stream);
+/* Adam: m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2; w -= lr_t m/(sqrt v+eps) */
+int gpd_adam_step(long n, float lr_t, float beta1, float beta2, float eps,
+                  float* d_w, const float* d_g, float* d_m, float* d_v,
+                  void* stream);
 
 #ifdef __cplusplus
 }

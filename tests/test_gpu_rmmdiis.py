@@ -22,7 +22,7 @@ def _setup(dtype, nb):
     from gpawdft_amd.eigensolvers import WaveFunctions
     from gpawdft_amd.hamiltonian import Hamiltonian
     from gpawdft_amd.synthetic import System
-    sysm = System(16, 0.3, 2, seed=11)
+    sysm = System(16, 0.3, 2, seed=11, unit_projectors=True)
     kin, pt = sysm.operators()
     H = dense_hamiltonian(sysm, pt)
     # the FD Hamiltonian with a symmetric dH is symmetric

@@ -246,7 +246,7 @@ def test_rmmdiis_restatement_converges_to_dense_spectrum():
         oracle_preconditioner
     from gpawdft_amd.lfc import LFC
     from gpawdft_amd.synthetic import System
-    sysm = System(12, 0.3, 2, seed=11)
+    sysm = System(12, 0.3, 2, seed=11, unit_projectors=True)
     pt = LFC(sysm.gd, [sysm.splines] * sysm.natoms)
     pt.set_positions(sysm.spos_ac)
     H = dense_hamiltonian(sysm, pt)

@@ -11,8 +11,12 @@
 //   backward bias   gb[o]     = sum_{b,p} gy[b,o,p]
 // with d_t = (t/9 - 1, (t/3)%3 - 1, t%3 - 1) and periodic wrap.  Weights are
 // FP32 (o, i, 27).  Each thread of the data kernels computes one point for
-// 16 output channels; the weight kernel gives each (o, i) pair a workgroup
-// that reduces its 27 sums in LDS (fixed order, no atomics).  FP32 with FMA.
+// 16 output channels.  The weight kernel splits the B*P points into chunks:
+// one workgroup per (o, i, chunk) reduces its 27 sums (and the bias sum) in
+// LDS, then the chunks are summed in order (fixed order, no atomics).  FP32
+// with FMA.
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace gpd {
@@ -85,62 +89,64 @@ __global__ __launch_bounds__(256) void conv3d_kernel(const float* __restrict__ x
     }
 }
 
-// gW[o,i,t] = sum_{b,p} gy[b,o,p] x[b,i,p+d_t]; one workgroup per (o, i)
+// Partial gW[o,i,t] = sum_{b,p in chunk} gy[b,o,p] x[b,i,p+d_t]: one
+// workgroup per (o, i, chunk of the B*P points), 27 sums reduced in LDS in
+// a fixed order; part[(chunk * Cout + o) * Cin + i][27].
 __global__ __launch_bounds__(256) void conv3d_wgrad_kernel(const float* __restrict__ x,
                                                           const float* __restrict__ gy,
-                                                          float* __restrict__ gW, ConvDims d)
+                                                          float* __restrict__ part,
+                                                          float* __restrict__ partb, ConvDims d,
+                                                          long chunk)
 {
     __shared__ float red[256][28];
-    const int P = d.D * d.H * d.W;
-    const int o = blockIdx.x / d.Cin, i = blockIdx.x % d.Cin;
+    const long P = (long)d.D * d.H * d.W;
+    const int oi = blockIdx.x;
+    const int o = oi / d.Cin, i = oi % d.Cin;
+    const long q0 = (long)blockIdx.y * chunk;
+    const long q1 = min((long)d.B * P, q0 + chunk);
     float acc[27];
 #pragma unroll
     for (int t = 0; t < 27; t++) acc[t] = 0.f;
-    for (int b = 0; b < d.B; b++) {
+    float accb = 0.f;   // bias gradient, kept by the i == 0 workgroups
+    for (long q = q0 + threadIdx.x; q < q1; q += blockDim.x) {
+        const int b = (int)(q / P);
+        const int p = (int)(q % P);
         const float* xi = x + ((long)b * d.Cin + i) * P;
-        const float* go = gy + ((long)b * d.Cout + o) * P;
-        for (int p = threadIdx.x; p < P; p += blockDim.x) {
-            const float g = go[p];
-            const int px = p % d.W, py = (p / d.W) % d.H, pz = p / (d.W * d.H);
+        const float g = gy[((long)b * d.Cout + o) * P + p];
+        accb += g;
+        const int px = p % d.W, py = (p / d.W) % d.H, pz = p / (d.W * d.H);
 #pragma unroll
-            for (int t = 0; t < 27; t++) {
-                const int z = wrapp(pz + t / 9 - 1, d.D);
-                const int yy = wrapp(py + (t / 3) % 3 - 1, d.H);
-                const int xx = wrapp(px + t % 3 - 1, d.W);
-                acc[t] = __builtin_fmaf(g, xi[(z * d.H + yy) * d.W + xx], acc[t]);
-            }
+        for (int t = 0; t < 27; t++) {
+            const int z = wrapp(pz + t / 9 - 1, d.D);
+            const int yy = wrapp(py + (t / 3) % 3 - 1, d.H);
+            const int xx = wrapp(px + t % 3 - 1, d.W);
+            acc[t] = __builtin_fmaf(g, xi[(z * d.H + yy) * d.W + xx], acc[t]);
         }
     }
 #pragma unroll
     for (int t = 0; t < 27; t++) red[threadIdx.x][t] = acc[t];
+    red[threadIdx.x][27] = accb;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s)
-            for (int t = 0; t < 27; t++) red[threadIdx.x][t] += red[threadIdx.x + s][t];
+            for (int t = 0; t < 28; t++) red[threadIdx.x][t] += red[threadIdx.x + s][t];
         __syncthreads();
     }
-    if (threadIdx.x < 27) gW[((long)o * d.Cin + i) * 27 + threadIdx.x] = red[0][threadIdx.x];
+    if (threadIdx.x < 27)
+        part[((long)blockIdx.y * d.Cout * d.Cin + oi) * 27 + threadIdx.x] = red[0][threadIdx.x];
+    if (threadIdx.x == 27 && i == 0 && partb)
+        partb[(long)blockIdx.y * d.Cout + o] = red[0][27];
 }
 
-// gb[o] = sum_{b,p} gy[b,o,p]; one workgroup per o
-__global__ __launch_bounds__(256) void conv3d_bgrad_kernel(const float* __restrict__ gy,
-                                                          float* __restrict__ gb, ConvDims d)
+// gW = sum over chunks of the partials, in chunk order
+__global__ void conv3d_wgrad_reduce_kernel(const float* __restrict__ part, float* __restrict__ gW,
+                                           long n, int nchunk)
 {
-    __shared__ float red[256];
-    const int P = d.D * d.H * d.W;
-    const int o = blockIdx.x;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
     float s = 0.f;
-    for (int b = 0; b < d.B; b++) {
-        const float* go = gy + ((long)b * d.Cout + o) * P;
-        for (int p = threadIdx.x; p < P; p += blockDim.x) s += go[p];
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int k = 128; k > 0; k >>= 1) {
-        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) gb[o] = red[0];
+    for (int c = 0; c < nchunk; c++) s += part[(long)c * n + k];
+    gW[k] = s;
 }
 
 // leaky ReLU and its gradient (slope a for x < 0), elementwise
@@ -230,15 +236,39 @@ extern "C" int gpd_conv3d_backward_weight(int B, int Cin, int Cout, int D, int H
 {
     GPD_REQUIRE(conv_dims_ok(B, Cin, Cout, D, H, W), GPD_ERR_ARG, "bad conv sizes");
     GPD_REQUIRE(d_x && d_gy && d_gw, GPD_ERR_ARG, "bad arrays");
+    GPD_REQUIRE((long)Cout * Cin <= (1L << 30), GPD_ERR_ARG, "too many channel pairs");
     ConvDims d = {B, Cin, Cout, D, H, W};
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(conv3d_wgrad_kernel, dim3(Cout * Cin), dim3(256), 0, st, d_x, d_gy, d_gw,
-                       d);
+    // split the B*P points so that about 2048 workgroups run, chunks of at
+    // least 4096 points
+    const long npts = (long)B * D * H * W;
+    const long pairs = (long)Cout * Cin;
+    long nchunk = std::max(1L, std::min((2048 + pairs - 1) / pairs, npts / 4096));
+    nchunk = std::min(nchunk, 65535L);
+    const long chunk = (npts + nchunk - 1) / nchunk;
+    nchunk = (npts + chunk - 1) / chunk;
+    const long n = pairs * 27;
+    float* part = nullptr;
+    hipError_t e = hipMallocAsync((void**)&part, sizeof(float) * (n + Cout) * nchunk, st);
+    if (e != hipSuccess) {
+        set_last_error(hipGetErrorString(e));
+        return GPD_ERR_HIP;
+    }
+    float* partb = part + n * nchunk;
+    hipLaunchKernelGGL(conv3d_wgrad_kernel, dim3((unsigned)pairs, (unsigned)nchunk), dim3(256), 0,
+                       st, d_x, d_gy, part, partb, d, chunk);
     int rc = last_error_to_status(hipGetLastError());
-    if (!rc && d_gb) {
-        hipLaunchKernelGGL(conv3d_bgrad_kernel, dim3(Cout), dim3(256), 0, st, d_gy, d_gb, d);
+    if (!rc) {
+        hipLaunchKernelGGL(conv3d_wgrad_reduce_kernel, dim3((unsigned)((n + 255) / 256)),
+                           dim3(256), 0, st, part, d_gw, n, (int)nchunk);
         rc = last_error_to_status(hipGetLastError());
     }
+    if (!rc && d_gb) {
+        hipLaunchKernelGGL(conv3d_wgrad_reduce_kernel, dim3((unsigned)((Cout + 255) / 256)),
+                           dim3(256), 0, st, partb, d_gb, (long)Cout, (int)nchunk);
+        rc = last_error_to_status(hipGetLastError());
+    }
+    (void)hipFreeAsync(part, st);
     return rc;
 }
 

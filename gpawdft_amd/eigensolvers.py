@@ -200,6 +200,12 @@ class RMMDIIS(Eigensolver):
         self.limit_lambda = limit_lambda
         self.trial_step = trial_step
         self.timings = {}
+        # called as hook(R_x, eps_x) with each block's residuals before the
+        # preconditioner (training data for the learned preconditioner)
+        self.residual_hook = None
+        # bands that kept the learned correction / bands offered one
+        self.learned_kept = 0
+        self.learned_offered = 0
 
     def _tick(self, key, t0):
         torch.cuda.synchronize()
@@ -239,15 +245,28 @@ class RMMDIIS(Eigensolver):
             if niter > 1:
                 psit_diis.view(B, niter, *psit_x.shape[1:])[:Bx, 0] = psit_x
                 R_diis.view(B, niter, *R_x.shape[1:])[:Bx, 0] = R_x
-            dpsit_x = pc(R_x, kpt)
+            ekin_x = pc.calculate_kinetic_energy(psit_x, kpt)
+            if ekin_x is None and getattr(pc, 'uses_eigenvalues', False):
+                ekin_x = eps_x
+            if self.residual_hook is not None:
+                self.residual_hook(R_x, eps_x)
             P_x = P_d[:Bx]
-            wfs.apply_pseudo_hamiltonian(ham, dpsit_x, dR_x)
-            wfs.project(dpsit_x, P_x)
-            self.calculate_residuals(kpt, wfs, ham, dpsit_x, P_x, eps_x,
-                                     dR_x)
-            RdR = dots(dR_x, R_x, dv)
-            dRdR = dots(dR_x, dR_x, dv)
-            lam_x = -RdR / dRdR
+            use_net = None
+            if hasattr(pc, 'candidates'):
+                # keep, per band, the candidate correction whose line
+                # search leaves the smaller residual (multigrid fallback)
+                dpsit_x, dR_x, lam_x, use_net = self._choose(
+                    pc.candidates(R_x, kpt, ekin_x), ham, wfs, kpt, eps_x,
+                    R_x, dR_x, P_x, dv)
+            else:
+                dpsit_x = pc(R_x, kpt, ekin_x)
+                wfs.apply_pseudo_hamiltonian(ham, dpsit_x, dR_x)
+                wfs.project(dpsit_x, P_x)
+                self.calculate_residuals(kpt, wfs, ham, dpsit_x, P_x, eps_x,
+                                         dR_x)
+                RdR = dots(dR_x, R_x, dv)
+                dRdR = dots(dR_x, dR_x, dv)
+                lam_x = -RdR / dRdR
             if self.limit_lambda:
                 upper = self.limit_lambda['upper']
                 lower = self.limit_lambda['lower']
@@ -268,18 +287,64 @@ class RMMDIIS(Eigensolver):
                 if nit > 1 or self.limit_lambda:
                     self._diis(pd[:Bx], rd[:Bx], nit, psit_x, R_x, dv)
                 if nit < niter - 1:
-                    dpsit_x = pc(R_x, kpt)
+                    dpsit_x = self._precondition(pc, R_x, kpt, ekin_x,
+                                                 use_net)
                     axpy(lam_x, dpsit_x, psit_x)
                     wfs.apply_pseudo_hamiltonian(ham, psit_x, R_x)
                     wfs.project(psit_x, P_x)
                     self.calculate_residuals(kpt, wfs, ham, psit_x, P_x,
                                              eps_x, R_x)
-            dpsit_x = pc(R_x, kpt)
+            dpsit_x = self._precondition(pc, R_x, kpt, ekin_x, use_net)
             if self.trial_step is not None:
                 lam_x = np.full(Bx, self.trial_step)
             axpy(lam_x, dpsit_x, psit_x)
         self._tick('rmm-diis', t0)
         return error, psit_nG
+
+    def _choose(self, cands, ham, wfs, kpt, eps_x, R_x, dR_x, P_x, dv):
+        """Line search for each candidate correction; per band keep the
+        one with the smallest |R + lam dR|^2 = RR - RdR^2 / dRdR (the
+        first candidate is the multigrid one)."""
+        RR = dots(R_x, R_x, dv)
+        best = None
+        for k, d in enumerate(cands):
+            dR = dR_x if k == 0 else torch.empty_like(dR_x)
+            wfs.apply_pseudo_hamiltonian(ham, d, dR)
+            wfs.project(d, P_x)
+            self.calculate_residuals(kpt, wfs, ham, d, P_x, eps_x, dR)
+            RdR = dots(dR, R_x, dv)
+            dRdR = dots(dR, dR, dv)
+            left = RR - RdR**2 / dRdR
+            lam = -RdR / dRdR
+            if best is None:
+                best = [d, dR, lam, left, np.zeros(len(lam), bool)]
+                continue
+            better = left < best[3]
+            idx = np.nonzero(better)[0]
+            if len(idx):
+                ti = torch.as_tensor(idx, device=d.device)
+                best[0][ti] = d[ti]
+                best[1][ti] = dR[ti]
+            best[2] = np.where(better, lam, best[2])
+            best[3] = np.where(better, left, best[3])
+            best[4] |= better
+        self.learned_offered += len(best[4])
+        self.learned_kept += int(best[4].sum())
+        if best[1] is not dR_x:
+            dR_x.copy_(best[1])
+        return best[0], dR_x, best[2], best[4]
+
+    @staticmethod
+    def _precondition(pc, R_x, kpt, ekin_x, use_net):
+        if use_net is None:
+            return pc(R_x, kpt, ekin_x)
+        d = pc.mg(R_x, kpt).clone()
+        idx = np.nonzero(use_net)[0]
+        if len(idx):
+            ti = torch.as_tensor(idx, device=d.device)
+            d[ti] += pc.learned_part(R_x[ti].contiguous(),
+                                     np.asarray(ekin_x)[idx])
+        return d
 
     def _diis(self, pd, rd, nit, psit_x, R_x, dv):
         """Per band: minimise |sum_i a_i R_i| with sum_i a_i = 1 over the

@@ -1,0 +1,206 @@
+"""Training data and training loop of the learned preconditioner.
+
+Pairs come from the eigensolver itself: RMM-DIIS with the multigrid
+preconditioner runs on seeded Gaussian-well systems, and every block's
+residuals R (with their band eigenvalues eps) are kept.  The target of the
+network is the part of the exact shifted inverse the V-cycle misses,
+
+    y = (-(T - eps_t)^-1 R - MG(R)) / rms(R),    eps_t = min(eps, -0.05),
+
+with (T - eps_t)^-1 applied in Fourier space from the stencil's own symbol
+(periodic grids: plane waves are its eigenvectors), so the target is exact
+to rounding.  torch.fft does that transform; it builds training data only
+and is not on the preconditioner's path.
+
+The loss is the preconditioner's own figure of merit, the relative
+residual of the shifted equation left by the correction,
+
+    |(T - eps_t)(MG(R) + s N) + R|^2 / |R|^2 = mean(((T - eps_t)(N - y))^2),
+
+so errors are weighted as the eigensolver sees them (high-frequency noise
+in N is amplified by T); (T - eps_t) is the library's own stencil.
+
+    python -m gpawdft_amd.precond_training --out gpawdft_amd/data/unet_precond.npz
+
+trains on 64^3 systems (the default) and writes the weights.
+"""
+import argparse
+import time
+
+import numpy as np
+import torch
+
+from gpawdft_amd.eigensolvers import KPoint, RMMDIIS, WaveFunctions
+from gpawdft_amd.hamiltonian import Hamiltonian
+from gpawdft_amd.learned import UNet, network_input
+from gpawdft_amd.preconditioner import Preconditioner
+from gpawdft_amd.synthetic import System, smooth_bands_device
+
+EPS_MAX = -0.05
+
+
+def stencil_symbol(kin, n_c, device):
+    """Eigenvalues of the FD operator kin on the periodic FFT grid:
+    sum_p c_p cos(2 pi f . o_p)."""
+    f = [torch.fft.fftfreq(int(n), device=device, dtype=torch.float64)
+         for n in n_c]
+    F0, F1, F2 = torch.meshgrid(*f, indexing='ij')
+    lam = torch.zeros_like(F0)
+    for c, o in zip(kin.coef_p, kin.offset_pc):
+        lam += float(c) * torch.cos(2 * np.pi * (F0 * int(o[0]) +
+                                                 F1 * int(o[1]) +
+                                                 F2 * int(o[2])))
+    return lam
+
+
+def shifted_inverse(R, eps_t, lam):
+    """-(T - eps_t)^-1 R per band (R: (B, n0, n1, n2))."""
+    F = torch.fft.fftn(R.to(torch.float64), dim=(-3, -2, -1))
+    e = torch.as_tensor(np.asarray(eps_t, np.float64), device=R.device)
+    F = F / (lam[None] - e.view(-1, 1, 1, 1))
+    return -torch.fft.ifftn(F, dim=(-3, -2, -1)).real
+
+
+def clamp_eps(eps):
+    return np.minimum(np.asarray(eps, np.float64), EPS_MAX)
+
+
+class PairCollector:
+    """Residual hook: stores network inputs and targets on the device."""
+
+    def __init__(self, kin, mg, v_G, n_c, limit):
+        self.mg = mg
+        self.v = torch.as_tensor(np.asarray(v_G, np.float32)).cuda()
+        self.lam = stencil_symbol(kin, n_c, 'cuda')
+        self.X = []
+        self.Y = []
+        self.E = []
+        self.count = 0
+        self.limit = limit
+
+    def __call__(self, R_x, eps_x):
+        if self.count >= self.limit:
+            return
+        R = R_x[:self.limit - self.count].to(torch.float32).contiguous()
+        eps_t = clamp_eps(eps_x[:R.shape[0]])
+        x, s = network_input(R, self.v, eps_t)
+        d_mg = self.mg(R.to(R_x.dtype), None).to(torch.float64)
+        y = (shifted_inverse(R, eps_t, self.lam) - d_mg) / \
+            s.to(torch.float64).view(-1, 1, 1, 1)
+        self.X.append(x)
+        self.Y.append(y.to(torch.float32)[:, None])
+        self.E.append(torch.as_tensor(eps_t, dtype=torch.float32,
+                                      device=x.device))
+        self.count += R.shape[0]
+
+
+def generate_pairs(N, h_ang, natoms, seeds, nbands=32, iters=20,
+                   limit=None, blocksize=8):
+    """Run FP32 RMM-DIIS (multigrid) on System(N, h, natoms, seed) for each
+    seed and collect (x, y) pairs.  Returns X (n, 2, N, N, N), Y (n, 1,
+    N, N, N), the clamped eigenvalues E (n,) on the device and the kinetic
+    operator of the grid (the same for every seed)."""
+    X, Y, E = [], [], []
+    per_seed = None if limit is None else -(-limit // len(seeds))
+    for seed in seeds:
+        sysm = System(N, h_ang, natoms, seed=seed)
+        kin, pt = sysm.operators()
+        wfs = WaveFunctions(sysm.gd, kin, pt, nbands, torch.float32)
+        ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+        psi = smooth_bands_device(nbands, sysm.gd.n_c, seed + 100,
+                                  dtype=torch.float32)
+        kpt = KPoint(psi)
+        es = RMMDIIS(blocksize=blocksize)
+        mg = Preconditioner(sysm.gd, kin, block=blocksize)
+        col = PairCollector(kin, mg, sysm.vt_G, sysm.gd.n_c,
+                            per_seed if per_seed else 1 << 62)
+        es.residual_hook = col
+        for it in range(iters):
+            es.iterate(ham, wfs, kpt)
+            if col.count >= col.limit:
+                break
+        X += col.X
+        Y += col.Y
+        E += col.E
+    return torch.cat(X), torch.cat(Y), torch.cat(E), kin
+
+
+def shifted_apply(kin, e, eps):
+    """(T - eps) e for a batch of grids e (B, n0, n1, n2), eps (B,)."""
+    out = torch.empty_like(e)
+    kin.apply(e, out)
+    out -= eps.view(-1, 1, 1, 1) * e
+    return out
+
+
+def train(net, X, Y, E, kin, steps, batch=8, lr=1e-3, seed=0, log=None):
+    """Adam on mean(((T - eps)(N(x) - y))^2); returns the loss history."""
+    g = torch.Generator(device='cpu')
+    g.manual_seed(seed)
+    n = X.shape[0]
+    hist = []
+    for step in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g).to(X.device)
+        x = X[idx].contiguous()
+        y = Y[idx].contiguous()
+        eps = E[idx].contiguous()
+        out = net.forward(x, keep=True)
+        diff = (out - y)[:, 0].contiguous()
+        Td = shifted_apply(kin, diff, eps)
+        loss = float(torch.mean(Td.double()**2))
+        grad = shifted_apply(kin, Td, eps)
+        grad *= 2.0 / diff.numel()
+        net.backward(grad[:, None])
+        net.adam_step(lr)
+        hist.append(loss)
+        if log and (step % 100 == 0 or step == steps - 1):
+            log('step %d loss %.4e' % (step, loss))
+    return hist
+
+
+def residual_reduction(net_or_none, mg, kin, X, lam, R, eps_t):
+    """|(T - eps) d + R| / |R| per band for d = MG(R) (+ the learned part),
+    evaluated in Fourier space with the stencil symbol."""
+    d = mg(R, None).to(torch.float64)
+    if net_or_none is not None:
+        s = torch.sqrt(torch.mean(R.double().reshape(R.shape[0], -1)**2, 1))
+        y = net_or_none.forward(X)[:, 0].to(torch.float64)
+        d = d + y * s.view(-1, 1, 1, 1)
+    e = torch.as_tensor(np.asarray(eps_t), device=R.device)
+    Fd = torch.fft.fftn(d, dim=(-3, -2, -1))
+    Td = torch.fft.ifftn(Fd * (lam[None] - e.view(-1, 1, 1, 1)),
+                         dim=(-3, -2, -1)).real
+    res = Td + R.to(torch.float64)
+    return (torch.linalg.vector_norm(res.reshape(R.shape[0], -1), dim=1) /
+            torch.linalg.vector_norm(R.double().reshape(R.shape[0], -1),
+                                     dim=1)).cpu().numpy()
+
+
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument('--N', type=int, default=64)
+    p.add_argument('--h', type=float, default=0.18)
+    p.add_argument('--atoms', type=int, default=10)
+    p.add_argument('--seeds', type=int, default=25)
+    p.add_argument('--pairs', type=int, default=20000)
+    p.add_argument('--steps', type=int, default=3000)
+    p.add_argument('--batch', type=int, default=8)
+    p.add_argument('--lr', type=float, default=1e-3)
+    p.add_argument('--out', default='gpawdft_amd/data/unet_precond.npz')
+    a = p.parse_args()
+    t0 = time.time()
+    X, Y, E, kin = generate_pairs(a.N, a.h, a.atoms,
+                                  list(range(1000, 1000 + a.seeds)),
+                                  limit=a.pairs)
+    print('pairs', X.shape[0], 'in %.1f s' % (time.time() - t0), flush=True)
+    net = UNet(seed=0)
+    t0 = time.time()
+    train(net, X, Y, E, kin, a.steps, a.batch, a.lr,
+          log=lambda m: print(m, flush=True))
+    torch.cuda.synchronize()
+    print('trained in %.1f s' % (time.time() - t0), flush=True)
+    net.save(a.out)
+
+
+if __name__ == '__main__':
+    main()

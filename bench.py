@@ -36,6 +36,13 @@ def parse():
                    help='skip the CPU oracle baseline')
     p.add_argument('--cpu-seconds', type=float, default=12.0,
                    help='target length of the CPU baseline sample')
+    p.add_argument('--no-ttt', action='store_true',
+                   help='skip the time-to-1e-8 comparison of the multigrid '
+                        'and learned preconditioners')
+    p.add_argument('--ttt-max-iter', type=int, default=400)
+    p.add_argument('--no-ragged', action='store_true',
+                   help='skip the ragged-batch graph measurement')
+    p.add_argument('--ragged-systems', type=int, default=48)
     p.add_argument('--copy-calibration', action='store_true',
                    help='also run a plain copy of one band block (for '
                         'counter calibration under rocprofv3 --pmc)')
@@ -160,6 +167,14 @@ def main():
         del psi, out, P, c, pc, R, d
         torch.cuda.empty_cache()
 
+    ragged = None
+    if not args.no_ragged:
+        ragged = ragged_batch(args.ragged_systems, args.steps)
+
+    ttt = None
+    if not args.no_ttt:
+        ttt = time_to_1e8(args.ttt_max_iter)
+
     # CPU baseline: the FP64 oracle (all host cores) on a bounded sample
     cpu = None
     if not args.no_cpu:
@@ -195,6 +210,8 @@ def main():
                      'alg_bytes_per_launch': f32['alg_bytes'],
                      'ms_per_launch': f32['ms_stencil']},
         'cpu_baseline': cpu,
+        'time_to_1e-8': ttt,
+        'ragged_batch': ragged,
         'f32_detail': {k: f32[k] for k in ('ms_stencil', 'ms_projections',
                                            'ms_projector_add', 'precond_ms',
                                            'precond_value')},
@@ -214,6 +231,62 @@ def main():
         line['hpsi_plus_precond_speedup_vs_cpu_f32'] = (
             f32['hpsi_plus_precond_value'] / cpu['hpsi_plus_precond_value'])
     print(json.dumps(line))
+
+
+def ragged_batch(nsys, steps):
+    """Throughput of one step (H psi + multigrid preconditioner, FP32) over
+    nsys independent systems (grids from {32,48,64,80}^3, 16-256 bands, 4-48
+    atoms): per-system eager launches versus one replay of a graph holding
+    the same launches in parallel branches, after the launch-shape
+    autotuner."""
+    import torch
+    from gpawdft_amd.ragged import RaggedBatch, random_specs
+    t0 = time.perf_counter()
+    rb = RaggedBatch(random_specs(nsys, seed=0), block=16)
+    setup = time.perf_counter() - t0
+    tuning = rb.autotune(reps=2)
+    rb.capture()
+    res = {}
+    for name, fn in (('sequential', rb.run_sequential), ('graph', rb.replay)):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        res[name] = {'ms_per_step': dt * 1e3,
+                     'band_gridpts_per_s': rb.band_points / dt}
+    res['graph_speedup'] = (res['sequential']['ms_per_step'] /
+                            res['graph']['ms_per_step'])
+    res['graph_branches'] = rb.nstreams
+    res['systems'] = nsys
+    res['band_gridpts_per_step'] = rb.band_points
+    res['setup_s'] = setup
+    res['wg_targets'] = {'%d pts x %d bands' % k: v for k, v in tuning.items()}
+    del rb
+    torch.cuda.empty_cache()
+    return res
+
+
+def time_to_1e8(max_iter):
+    """Wall-clock seconds for FP64 RMM-DIIS to bring the occupied-band
+    residual error below 1e-8 with the multigrid preconditioner and with the
+    learned one (multigrid fallback), on a held-out 32^3 system of the same
+    generators (h = 0.18 A, 3 atoms, 16 bands, 12 occupied), same start."""
+    from gpawdft_amd.learned import UNet
+    from gpawdft_amd.precond_training import time_to_tolerance
+    from gpawdft_amd.synthetic import System
+    net = UNet.load(os.path.join(ROOT, 'gpawdft_amd', 'data',
+                                 'unet_precond.npz'))
+    sysm = System(32, 0.18, 3, seed=4242)
+    out = {'workload': '32^3 periodic grid, h=0.18 A, 3 atoms, 16 FP64 '
+                       'bands (12 occupied), blocksize 8, tolerance 1e-8, '
+                       'at most %d iterations' % max_iter}
+    for name, n in (('multigrid', None), ('learned', net)):
+        out[name] = time_to_tolerance(sysm, n, nbands=16, nocc=12,
+                                      max_iter=max_iter)
+    return out
 
 
 def measured_traffic():

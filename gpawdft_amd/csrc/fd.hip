@@ -326,12 +326,12 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
     return cf;
 }
 
-static int pick_tz(int n0, long work_per_plane_chunk, int R)
+static int pick_tz(int n0, long work_per_plane_chunk, int R, long target)
 {
-    // Split axis 0 into chunks until there are ~8192 workgroups (about four
-    // rounds of the 2048 that fit on the chip), keeping chunks at least
-    // max(16, 4R) planes deep so the window warm-up stays cheap.
-    const long target = 8192;
+    // Split axis 0 into chunks until there are about `target` workgroups
+    // (8192 by default: four rounds of the 2048 that fit on the chip),
+    // keeping chunks at least max(16, 4R) planes deep so the window warm-up
+    // stays cheap.
     const int min_depth = std::max(16, 4 * R);
     long nz = (target + work_per_plane_chunk - 1) / std::max<long>(1, work_per_plane_chunk);
     nz = std::max<long>(1, std::min<long>(nz, n0 / min_depth));
@@ -344,7 +344,7 @@ static int launch_axis_R(const gpd_operator* op, long nb, const T* in, T* out,
 {
     const int n0 = op->n[0], n1 = op->n[1], n2 = op->n[2];
     const int tiles1 = (n1 + kTY - 1) / kTY, tiles2 = (n2 + kTX - 1) / kTX;
-    const int tz = pick_tz(n0, nb * tiles1 * tiles2, R);
+    const int tz = pick_tz(n0, nb * tiles1 * tiles2, R, op->wg_target);
     const int zchunks = (n0 + tz - 1) / tz;
     const long nwork = nb * zchunks * (long)tiles1 * tiles2;
     AxisCoefs<T, R> cf = make_axis_coefs<T, R>(op);
@@ -614,6 +614,14 @@ extern "C" int gpd_operator_apply(gpd_operator* op, int dtype, long nin, const v
         return fd_apply_typed<float>(op, EPI_PLAIN, nin, (const float*)d_in, (float*)d_out, ep, st);
     }
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_operator_set_launch(gpd_operator* op, long wg_target)
+{
+    GPD_REQUIRE(op, GPD_ERR_ARG, "null operator");
+    GPD_REQUIRE(wg_target >= 1 && wg_target <= (1L << 24), GPD_ERR_ARG, "bad workgroup target");
+    op->wg_target = wg_target;
+    return GPD_OK;
 }
 
 extern "C" int gpd_operator_get_diagonal_element(const gpd_operator* op, double* diag)

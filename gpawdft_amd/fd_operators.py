@@ -101,6 +101,12 @@ class FDOperator:
               'Operator.apply')
         vout.finish()
 
+    def set_launch(self, wg_target):
+        """Launch shape of the streaming stencil (about wg_target
+        workgroups); results do not depend on it."""
+        check(lib().gpd_operator_set_launch(self._handle, int(wg_target)),
+              'Operator.set_launch')
+
     def relax(self, relax_method, f_g, s_g, n, w=None):
         raise NotImplementedError('relaxation (Poisson solver) is not part '
                                   'of this build')

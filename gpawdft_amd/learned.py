@@ -106,7 +106,7 @@ class UNet:
                 for k in self.offsets}
 
     def save(self, path):
-        np.savez(path, **{k.replace('.', '__'): v
+        np.savez(path, **{k.replace('.', '__'): v.astype(np.float32)
                           for k, v in self.params_numpy().items()})
 
     @classmethod

@@ -95,7 +95,7 @@ class PairCollector:
 
 
 def generate_pairs(N, h_ang, natoms, seeds, nbands=32, iters=20,
-                   limit=None, blocksize=8):
+                   limit=None, blocksize=8, log=None):
     """Run FP32 RMM-DIIS (multigrid) on System(N, h, natoms, seed) for each
     seed and collect (x, y) pairs.  Returns X (n, 2, N, N, N), Y (n, 1,
     N, N, N), the clamped eigenvalues E (n,) on the device and the kinetic
@@ -122,6 +122,8 @@ def generate_pairs(N, h_ang, natoms, seeds, nbands=32, iters=20,
         X += col.X
         Y += col.Y
         E += col.E
+        if log:
+            log('seed %d: %d pairs' % (seed, col.count))
     return torch.cat(X), torch.cat(Y), torch.cat(E), kin
 
 
@@ -176,6 +178,42 @@ def residual_reduction(net_or_none, mg, kin, X, lam, R, eps_t):
                                      dim=1)).cpu().numpy()
 
 
+def time_to_tolerance(sysm, net, nbands=16, nocc=12, tol=1e-8,
+                      max_iter=150, dtype=torch.float64, seed=5, blocksize=8):
+    """Wall-clock seconds and iterations RMM-DIIS needs to bring the error
+    sum_n f_n <R_n|R_n> (f_n = 1 for the lowest nocc bands, 0 above: the
+    reference's convergence over occupied bands) below tol, with the
+    multigrid preconditioner (net None) or the learned one with multigrid
+    fallback.  Starts from smoothed random bands."""
+    from gpawdft_amd.learned import LearnedPreconditioner
+    kin, pt = sysm.operators()
+    ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
+    factory = None
+    if net is not None:
+        def factory(w, b):
+            return LearnedPreconditioner(
+                Preconditioner(sysm.gd, kin, block=b), net, sysm.vt_G)
+    wfs = WaveFunctions(sysm.gd, kin, pt, nbands, dtype,
+                        preconditioner_factory=factory)
+    f_n = np.zeros(nbands)
+    f_n[:nocc] = 1.0
+    kpt = KPoint(smooth_bands_device(nbands, sysm.gd.n_c, seed, dtype=dtype),
+                 f_n=f_n)
+    es = RMMDIIS(blocksize=blocksize)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    errors = []
+    for it in range(max_iter):
+        errors.append(es.iterate(ham, wfs, kpt))
+        if errors[-1] < tol:
+            break
+    torch.cuda.synchronize()
+    return {'seconds': time.perf_counter() - t0, 'iterations': len(errors),
+            'converged': bool(errors[-1] < tol), 'error': errors[-1],
+            'learned_kept': es.learned_kept,
+            'learned_offered': es.learned_offered}
+
+
 def main():
     p = argparse.ArgumentParser()
     p.add_argument('--N', type=int, default=64)
@@ -186,20 +224,44 @@ def main():
     p.add_argument('--steps', type=int, default=3000)
     p.add_argument('--batch', type=int, default=8)
     p.add_argument('--lr', type=float, default=1e-3)
+    p.add_argument('--iters', type=int, default=30)
     p.add_argument('--out', default='gpawdft_amd/data/unet_precond.npz')
     a = p.parse_args()
     t0 = time.time()
+    log = lambda m: print(m, flush=True)
     X, Y, E, kin = generate_pairs(a.N, a.h, a.atoms,
                                   list(range(1000, 1000 + a.seeds)),
-                                  limit=a.pairs)
+                                  iters=a.iters, limit=a.pairs, log=log)
     print('pairs', X.shape[0], 'in %.1f s' % (time.time() - t0), flush=True)
     net = UNet(seed=0)
     t0 = time.time()
-    train(net, X, Y, E, kin, a.steps, a.batch, a.lr,
-          log=lambda m: print(m, flush=True))
+    train(net, X, Y, E, kin, a.steps, a.batch, a.lr, log=log)
     torch.cuda.synchronize()
     print('trained in %.1f s' % (time.time() - t0), flush=True)
     net.save(a.out)
+    del X, Y, E
+    # held-out systems: relative shifted-equation residual left by the
+    # V-cycle alone and with the learned part
+    Xh, Yh, Eh, kin = generate_pairs(a.N, a.h, a.atoms, [2000, 2001],
+                                     iters=a.iters, limit=256, log=log)
+    with torch.no_grad():
+        zero = torch.zeros_like(Yh[:, 0])
+        r_mg, r_net = [], []
+        for b0 in range(0, Xh.shape[0], a.batch):
+            sl = slice(b0, b0 + a.batch)
+            y = Yh[sl, 0].contiguous()
+            e = Eh[sl].contiguous()
+            # N = 0 is the V-cycle alone
+            rm = shifted_apply(kin, (zero[sl] - y).contiguous(), e)
+            out = net.forward(Xh[sl].contiguous())[:, 0]
+            rn = shifted_apply(kin, (out - y).contiguous(), e)
+            # |(T - eps) d + R| / |R| with |R|^2 = npts rms^2
+            r_mg += list(torch.sqrt(torch.mean(rm.double()**2, (1, 2, 3)))
+                         .cpu().numpy())
+            r_net += list(torch.sqrt(torch.mean(rn.double()**2, (1, 2, 3)))
+                          .cpu().numpy())
+    print('held-out relative residual: multigrid %.4f, learned %.4f'
+          % (np.mean(r_mg), np.mean(r_net)), flush=True)
 
 
 if __name__ == '__main__':

@@ -329,7 +329,8 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
 static int pick_tz(int n0, long work_per_plane_chunk, int R, long target)
 {
     // Split axis 0 into chunks until there are about `target` workgroups
-    // (8192 by default: four rounds of the 2048 that fit on the chip),
+    // (16384 by default: eight rounds of the 2048 that fit on the chip;
+    // measured 5 % faster than 8192 at 96^3 x 256),
     // keeping chunks at least max(16, 4R) planes deep so the window warm-up
     // stays cheap.
     const int min_depth = std::max(16, 4 * R);

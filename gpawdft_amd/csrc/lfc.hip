@@ -90,15 +90,17 @@ __global__ void lfc_integrate_seq_kernel(const T* __restrict__ a, T* __restrict_
         if (m < nm) cb[m] = acc[m];
 }
 
-// Grouped integrate: one workgroup per (group, block of NBB bands).
-template <typename T, int NBB>
+// Grouped integrate: one workgroup per (group, block of NBB bands).  NF is
+// the largest function count of a group the instantiation holds in
+// registers (9: one atom with l = 0..2; 16 otherwise).
+template <typename T, int NBB, int NF>
 __global__ __launch_bounds__(256) void lfc_integrate_grp_kernel(
     const T* __restrict__ a, T* __restrict__ c, long nG, int Mmax, long nx, T dv,
     int ngroups, int nfmax, const long* __restrict__ grp_off, const int* __restrict__ grp_nf,
     const int* __restrict__ grp_fM, const int* __restrict__ grp_G,
     const float* __restrict__ grp_A)
 {
-    __shared__ float red[4][NBB * kNF];
+    __shared__ float red[4][NBB * NF];
     const long nbb = (nx + NBB - 1) / NBB;
     const long w = xcd_work_index(blockIdx.x, nbb * ngroups);
     if (w < 0) return;
@@ -107,40 +109,43 @@ __global__ __launch_bounds__(256) void lfc_integrate_grp_kernel(
     const int nf = grp_nf[g];
     const long j0 = grp_off[g], j1 = grp_off[g + 1];
 
-    float acc[NBB][kNF];
+    float acc[NBB][NF];
 #pragma unroll
     for (int bb = 0; bb < NBB; bb++)
 #pragma unroll
-        for (int f = 0; f < kNF; f++) acc[bb][f] = 0.f;
+        for (int f = 0; f < NF; f++) acc[bb][f] = 0.f;
 
     for (long j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
         const long G = grp_G[j];
         const float* Ar = grp_A + j * nfmax;
-        float av[kNF];
-#pragma unroll
-        for (int f = 0; f < kNF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+        float x[NBB];
 #pragma unroll
         for (int bb = 0; bb < NBB; bb++) {
             const long band = b0 + bb;
-            const float x = band < nx ? (float)a[band * nG + G] : 0.f;
-#pragma unroll
-            for (int f = 0; f < kNF; f++) acc[bb][f] = __builtin_fmaf(x, av[f], acc[bb][f]);
+            x[bb] = band < nx ? (float)a[band * nG + G] : 0.f;
         }
+        float av[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+#pragma unroll
+        for (int bb = 0; bb < NBB; bb++)
+#pragma unroll
+            for (int f = 0; f < NF; f++) acc[bb][f] = __builtin_fmaf(x[bb], av[f], acc[bb][f]);
     }
     // reduce over the wave, then over the 4 waves
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
 #pragma unroll
     for (int bb = 0; bb < NBB; bb++)
 #pragma unroll
-        for (int f = 0; f < kNF; f++) {
+        for (int f = 0; f < NF; f++) {
             float v = acc[bb][f];
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-            if (lane == 0) red[wave][bb * kNF + f] = v;
+            if (lane == 0) red[wave][bb * NF + f] = v;
         }
     __syncthreads();
-    for (int t = threadIdx.x; t < NBB * kNF; t += blockDim.x) {
-        const int bb = t / kNF, f = t % kNF;
+    for (int t = threadIdx.x; t < NBB * NF; t += blockDim.x) {
+        const int bb = t / NF, f = t % NF;
         const long band = b0 + bb;
         if (f < nf && band < nx) {
             float s = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
@@ -153,8 +158,10 @@ __global__ __launch_bounds__(256) void lfc_integrate_grp_kernel(
 // Grouped add (FP32): one thread per point of a group, NB bands per
 // workgroup; all functions of the group are summed, then added once.
 // Groups of one launch ("colour") cover disjoint points, so the
-// read-modify-writes do not race.
-template <int NB>
+// read-modify-writes do not race.  The NB x nf coefficients of the
+// workgroup are staged in LDS once; each thread issues its NB loads
+// together, then the sums, then the NB stores.
+template <int NB, int NF>
 __global__ __launch_bounds__(256) void lfc_add_grp_kernel(
     const float* __restrict__ c, float* __restrict__ a, long nG, int Mmax, long nx, int nfmax,
     int nitems, const int* __restrict__ item_g, const long* __restrict__ item_j0,
@@ -162,32 +169,46 @@ __global__ __launch_bounds__(256) void lfc_add_grp_kernel(
     const int* __restrict__ grp_fM, const int* __restrict__ grp_G,
     const float* __restrict__ grp_A)
 {
+    static_assert(NB * NF <= 256, "one coefficient per thread");
+    __shared__ float cs[NB][NF];
     const long nbblk = (nx + NB - 1) / NB;
     const long w = xcd_work_index(blockIdx.x, (long)nitems * nbblk);
     if (w < 0) return;
     const int item = (int)(w % nitems);
     const long b0 = (w / nitems) * NB;
     const int g = item_g[item];
+    const int nf = grp_nf[g];
+    const int* fM = grp_fM + (long)g * nfmax;
+    if ((int)threadIdx.x < NB * NF) {
+        const int bb = threadIdx.x / NF, f = threadIdx.x % NF;
+        const long b = b0 + bb;
+        cs[bb][f] = (f < nf && b < nx) ? c[b * Mmax + fM[f]] : 0.f;
+    }
+    __syncthreads();
     const long j = item_j0[item] + threadIdx.x;
     if (j >= grp_off[g + 1]) return;
-    const int nf = grp_nf[g];
     const long G = grp_G[j];
     const float* Ar = grp_A + j * nfmax;
-    const int* fM = grp_fM + (long)g * nfmax;
-    float av[kNF];
-#pragma unroll
-    for (int f = 0; f < kNF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+    float cur[NB];
 #pragma unroll
     for (int bb = 0; bb < NB; bb++) {
         const long b = b0 + bb;
-        if (b >= nx) break;
-        const float* cb = c + b * Mmax;
+        cur[bb] = b < nx ? a[b * nG + G] : 0.f;
+    }
+    float av[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) av[f] = f < nf ? Ar[f] : 0.f;
+#pragma unroll
+    for (int bb = 0; bb < NB; bb++) {
         float s = 0.f;
 #pragma unroll
-        for (int f = 0; f < kNF; f++)
-            if (f < nf) s = __builtin_fmaf(av[f], cb[fM[f]], s);
-        float* ap = a + b * nG + G;
-        *ap = *ap + s;
+        for (int f = 0; f < NF; f++) s = __builtin_fmaf(av[f], cs[bb][f], s);
+        cur[bb] = cur[bb] + s;
+    }
+#pragma unroll
+    for (int bb = 0; bb < NB; bb++) {
+        const long b = b0 + bb;
+        if (b < nx) a[b * nG + G] = cur[bb];
     }
 }
 
@@ -339,17 +360,18 @@ static hipError_t upload(V** dst, const V* src, size_t n)
     return hipMemset(*dst, 0, sizeof(V) * (n ? n : 1));
 }
 
-int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st)
+template <int NB, int NF>
+static int lfc_add_grouped_launch(const gpd_lfc* l, long nx, const float* c, float* a,
+                                  hipStream_t st)
 {
-    constexpr int NB = 8;
     const long nbblk = (nx + NB - 1) / NB;
     const LfcGroups& gs = l->ga;
     for (int col = 0; col < gs.ncolours; col++) {
         const int i0 = gs.colour_items[col], i1 = gs.colour_items[col + 1];
         if (i1 == i0) continue;
         const long nwork = (long)(i1 - i0) * nbblk;
-        hipLaunchKernelGGL(lfc_add_grp_kernel<NB>, dim3(xcd_grid_size(nwork)), dim3(256), 0, st, c,
-                           a, l->nG, l->Mmax, nx, gs.nfmax, i1 - i0, gs.item_g + i0,
+        hipLaunchKernelGGL((lfc_add_grp_kernel<NB, NF>), dim3(xcd_grid_size(nwork)), dim3(256), 0,
+                           st, c, a, l->nG, l->Mmax, nx, gs.nfmax, i1 - i0, gs.item_g + i0,
                            gs.item_j0 + i0, gs.grp_off, gs.grp_nf, gs.grp_fM, gs.grp_G,
                            gs.grp_A32);
         int rc = last_error_to_status(hipGetLastError());
@@ -357,6 +379,15 @@ int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hip
     }
     return GPD_OK;
 }
+
+int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st)
+{
+    // 16 bands per workgroup share each point's A row when groups hold at
+    // most 9 functions (one atom with l = 0..2)
+    if (l->ga.nfmax <= 9) return lfc_add_grouped_launch<16, 9>(l, nx, c, a, st);
+    return lfc_add_grouped_launch<8, kNF>(l, nx, c, a, st);
+}
+
 
 
 // Groups of keys with identical point lists (<= kNF functions each, keys
@@ -599,15 +630,26 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
         return last_error_to_status(hipGetLastError());
     }
     if (dtype == GPD_F32) {
-        constexpr int NBB = 4;
         const LfcGroups& gs = l->gi;
-        long nwork = ((nx + NBB - 1) / NBB) * gs.ngroups;
-        hipLaunchKernelGGL((lfc_integrate_grp_kernel<float, NBB>), dim3(xcd_grid_size(nwork)),
-                           dim3(256), 0, st, (const float*)d_a_xG, (float*)d_c_xM, l->nG, l->Mmax,
-                           nx, (float)l->dv, gs.ngroups, gs.nfmax, gs.grp_off, gs.grp_nf,
-                           gs.grp_fM, gs.grp_G, gs.grp_A32);
+        if (gs.nfmax <= 9) {
+            // 8 bands per workgroup share each point's A row
+            constexpr int NBB = 8;
+            long nwork = ((nx + NBB - 1) / NBB) * gs.ngroups;
+            hipLaunchKernelGGL((lfc_integrate_grp_kernel<float, NBB, 9>),
+                               dim3(xcd_grid_size(nwork)), dim3(256), 0, st, (const float*)d_a_xG,
+                               (float*)d_c_xM, l->nG, l->Mmax, nx, (float)l->dv, gs.ngroups,
+                               gs.nfmax, gs.grp_off, gs.grp_nf, gs.grp_fM, gs.grp_G, gs.grp_A32);
+        } else {
+            constexpr int NBB = 4;
+            long nwork = ((nx + NBB - 1) / NBB) * gs.ngroups;
+            hipLaunchKernelGGL((lfc_integrate_grp_kernel<float, NBB, kNF>),
+                               dim3(xcd_grid_size(nwork)), dim3(256), 0, st, (const float*)d_a_xG,
+                               (float*)d_c_xM, l->nG, l->Mmax, nx, (float)l->dv, gs.ngroups,
+                               gs.nfmax, gs.grp_off, gs.grp_nf, gs.grp_fM, gs.grp_G, gs.grp_A32);
+        }
         return last_error_to_status(hipGetLastError());
     }
+
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
 }
 

@@ -62,7 +62,7 @@ int gpd_operator_apply(gpd_operator* op, int dtype, long nin,
 int gpd_operator_get_diagonal_element(const gpd_operator* op, double* diag);
 
 /* Launch shape of the streaming stencil: split axis 0 into chunks until
- * about wg_target workgroups run (default 8192).  No counterpart in the
+ * about wg_target workgroups run (default 16384).  No counterpart in the
  * reference (its stencil is a serial loop); the ragged-batch autotuner sets
  * it per grid and band count.  Results do not depend on it. */
 int gpd_operator_set_launch(gpd_operator* op, long wg_target);

@@ -5,6 +5,8 @@ pattern is not pinned; the tolerances are written per test)."""
 import numpy as np
 import pytest
 
+import oracle as O
+
 pytestmark = pytest.mark.gpu
 
 
@@ -169,3 +171,66 @@ def test_matrix_operator_orthonormalize_and_diag(cuda):
     op.matrix_multiply(H, psi, P, out_nG=psi)
     H2 = op.calculate_matrix_elements(psi, P, Hop, dH, pt=pt).cpu().numpy()
     assert np.abs(H2 - np.diag(eps)).max() < 1e-8 * np.abs(eps).max()
+
+
+def test_mixed_precision_subspace_step_112x512(cuda):
+    """The mixed-precision subspace step at its full size: 112^3 grid x 512
+    bands, 64 atoms.  FP32 H psi, split-BF16 S and H on MFMA, FP64 inverse
+    Cholesky, FP32 rotation and FP64 eigensolve on the device
+    (Overlap.orthonormalize + Eigensolver.subspace_diagonalize).  Checked
+    against FP64 numpy/scipy on the same FP32 inputs, with the oracle's FP64
+    H psi: S to 1e-6 relative (Frobenius), eigenvalues of the pencil
+    (H, S) to 2e-6 of the largest."""
+    import scipy.linalg
+    import torch
+    from gpawdft_amd.eigensolvers import KPoint, RMMDIIS, WaveFunctions
+    from gpawdft_amd.hamiltonian import Hamiltonian, unpack
+    from gpawdft_amd.hs_operators import BandLayouts, MatrixOperator
+    from gpawdft_amd.synthetic import System, smooth_bands_device
+    sysm = System(112, 0.18, 64, seed=31)
+    gd = sysm.gd
+    kin, pt = sysm.operators()
+    ham = Hamiltonian(gd, [sysm.vt_G], sysm.dH_asp)
+    N = 512
+    psi = smooth_bands_device(N, gd.n_c, 4, dtype=torch.float32)
+    # S of the FP32 bands on the device (split-BF16 MFMA)
+    P = torch.zeros((N, pt.Mmax), dtype=torch.float32, device='cuda')
+    pt.integrate_xM(psi, P)
+    op = MatrixOperator(BandLayouts(gd, N))
+    S32 = op.calculate_matrix_elements(psi, None, lambda x: x, None)
+    S32 = S32.cpu().numpy().copy()
+    # the device subspace step
+    wfs = WaveFunctions(gd, kin, pt, N, torch.float32)
+    kpt = KPoint(psi.clone())
+    wfs.overlap.orthonormalize(wfs, kpt)
+    wfs.orthonormalized = True
+    es = RMMDIIS()
+    es.initialize(wfs)
+    es.subspace_diagonalize(ham, wfs, kpt)
+    eps32 = np.array(kpt.eps_n)
+    del wfs, kpt, es
+    # FP64 reference on the host, same inputs
+    x = psi.double().cpu().numpy()
+    del psi, P
+    torch.cuda.empty_cache()
+    Hx = O.fd(kin.coef_p, kin.offset_pc, gd.pbc_c, x)
+    Hx = O.add_local_potential(x, sysm.vt_G, Hx)
+    P64 = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, x,
+                          pt.Mmax)
+    c64 = O.projector_coefs(P64, [(pt.M_a[a], unpack(sysm.dH_asp[a][0]))
+                                  for a in pt.atom_indices])
+    Hx = O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c64, Hx)
+    x2 = x.reshape(N, -1)
+    S64 = gd.dv * (x2 @ x2.T)
+    H64 = gd.dv * (x2 @ Hx.reshape(N, -1).T)
+    del x, x2, Hx
+    H64 = 0.5 * (H64 + H64.T)
+    eps64 = scipy.linalg.eigh(H64, S64, eigvals_only=True)
+    S32 = np.tril(S32) + np.tril(S32, -1).T
+    s_err = np.linalg.norm(S32 - S64) / np.linalg.norm(S64)
+    e_err = np.abs(eps32 - eps64).max() / np.abs(eps64).max()
+    print('[subspace 112^3 x 512] |S-S64|/|S64| = %.3e, '
+          'max|eps-eps64|/max|eps64| = %.3e, eps in [%.3f, %.3f]'
+          % (s_err, e_err, eps64.min(), eps64.max()))
+    assert s_err < 1e-6
+    assert e_err < 2e-6

@@ -75,6 +75,8 @@ _SIGNATURES = {
     'gpd_diagonalize': (_i, [_i, _vp, _vp, _c_int_p, _vp]),
     'gpd_axpy_batched': (_i, [_i, _l, _l, _vp, _vp, _vp, _vp]),
     'gpd_dots': (_i, [_i, _l, _l, _d, _vp, _vp, _vp, _vp]),
+    'gpd_conv3d_forward_bf16': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp,
+                                    _vp, _vp]),
     'gpd_conv3d_forward': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                 _i, _vp]),
     'gpd_conv3d_backward_data': (_i, [_i, _i, _i, _i, _i, _i, _vp, _vp, _vp,

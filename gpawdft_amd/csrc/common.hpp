@@ -63,6 +63,24 @@ inline long xcd_grid_size(long nwork)
 
 void set_last_error(const char* msg);
 
+// MFMA operand and accumulator vectors
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// round-to-nearest-even float -> bfloat16 bits (finite inputs)
+__device__ __forceinline__ unsigned short bf16_bits(float x)
+{
+    unsigned int u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_value(unsigned short b)
+{
+    return __uint_as_float(((unsigned int)b) << 16);
+}
+
+
 }  // namespace gpd
 
 #define GPD_CHECK_HIP(expr)                                   \

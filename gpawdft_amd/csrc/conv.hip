@@ -310,3 +310,155 @@ extern "C" int gpd_scale_add(long n, float alpha, const float* d_x, float beta, 
                        n, alpha, d_x, beta, d_y);
     return last_error_to_status(hipGetLastError());
 }
+
+// ------------------------------------------------------------------ //
+// Forward convolution on MFMA with bfloat16 operands (inference).
+//
+// Implicit GEMM per workgroup tile of 256 output points (one z-plane, TY
+// rows, TX columns; TX = 32 when 32 divides W, else 16): D[o][p] =
+// sum_{t, c} W[o, c, t] x[c, p + d_t] on v_mfma_f32_32x32x8_bf16 with
+// A = weights (rows o), B = gathered inputs (columns p), k = 8 channels at
+// one tap.  Input channels are staged 8 at a time: the tile's input points
+// with their 1-point halo (3 planes) and the weights of those channels go
+// to LDS as bfloat16 (round to nearest even); the FP32 accumulators hold
+// all 27 taps and all channels.  Each wave owns 64 points (two 32-point
+// column blocks) for NOB blocks of 32 output channels.  Requires Cin % 8 ==
+// 0, Cout <= 32 NOB, W % 16 == 0 and H divisible by TY.
+namespace gpd {
+
+constexpr int kMCh = 8;   // channels per LDS stage (one MFMA k-step)
+
+template <int TX, int NOB>
+__global__ __launch_bounds__(256) void conv3d_fwd_mfma_kernel(const float* __restrict__ x,
+                                                             const float* __restrict__ Wt,
+                                                             const float* __restrict__ bias,
+                                                             float* __restrict__ y, ConvDims d)
+{
+    constexpr int TY = 256 / TX;
+    constexpr int HX = TX + 2, HY = TY + 2;
+    constexpr int NPTS = 3 * HY * HX;
+    constexpr int NO = 32 * NOB;
+    __shared__ __align__(16) short xs[NPTS * kMCh];
+    __shared__ __align__(16) short ws[27 * NO * kMCh];
+
+    const int P = d.D * d.H * d.W;
+    const int ntx = d.W / TX, nty = d.H / TY;
+    const long nwork = (long)d.B * d.D * nty * ntx;
+    const long w = xcd_work_index(blockIdx.x, nwork);
+    if (w < 0) return;
+    const int tx_i = (int)(w % ntx);
+    long r = w / ntx;
+    const int ty_i = (int)(r % nty);
+    r /= nty;
+    const int z = (int)(r % d.D);
+    const int b = (int)(r / d.D);
+    const int x0 = tx_i * TX, y0 = ty_i * TY;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+
+    f32x16 acc[2][NOB];
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++)
+#pragma unroll
+        for (int ob = 0; ob < NOB; ob++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[pb][ob][v] = 0.f;
+
+    // this lane's output points (column of the B operand) in each block
+    int qy[2], qx[2];
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++) {
+        const int q = wave * 64 + pb * 32 + lane % 32;
+        qy[pb] = q / TX;
+        qx[pb] = q % TX;
+    }
+    const int cq = 4 * (lane / 32);   // this lane's 4 channels of the k-step
+
+    for (int c0 = 0; c0 < d.Cin; c0 += kMCh) {
+        __syncthreads();
+        for (int s = threadIdx.x; s < NPTS * kMCh; s += 256) {
+            const int c = s / NPTS, pt = s % NPTS;
+            const int hz = pt / (HY * HX), rem = pt % (HY * HX);
+            const int hy = rem / HX, hx = rem % HX;
+            const int gz = wrapp(z - 1 + hz, d.D);
+            const int gy = wrapp(y0 - 1 + hy, d.H);
+            const int gx = wrapp(x0 - 1 + hx, d.W);
+            const float v = x[((long)b * d.Cin + c0 + c) * P + ((long)gz * d.H + gy) * d.W + gx];
+            xs[pt * kMCh + c] = (short)bf16_bits(v);
+        }
+        for (int s = threadIdx.x; s < 27 * NO * kMCh; s += 256) {
+            const int c = s % kMCh, rem = s / kMCh;
+            const int o = rem % NO, t = rem / NO;
+            const float v = o < d.Cout ? Wt[((long)o * d.Cin + c0 + c) * 27 + t] : 0.f;
+            ws[(t * NO + o) * kMCh + c] = (short)bf16_bits(v);
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int t = 0; t < 27; t++) {
+            const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;
+            bf16x4 bop[2];
+#pragma unroll
+            for (int pb = 0; pb < 2; pb++) {
+                const int pt = (dz * HY + qy[pb] + dy) * HX + qx[pb] + dx;
+                bop[pb] = *reinterpret_cast<const bf16x4*>(&xs[pt * kMCh + cq]);
+            }
+#pragma unroll
+            for (int ob = 0; ob < NOB; ob++) {
+                const int o = ob * 32 + lane % 32;
+                const bf16x4 aop = *reinterpret_cast<const bf16x4*>(&ws[(t * NO + o) * kMCh + cq]);
+#pragma unroll
+                for (int pb = 0; pb < 2; pb++)
+                    acc[pb][ob] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(aop, bop[pb],
+                                                                          acc[pb][ob], 0, 0, 0);
+            }
+        }
+    }
+    // D: column = point lane % 32, row = output channel 8 (v / 4) + 4 (lane / 32) + v % 4
+#pragma unroll
+    for (int pb = 0; pb < 2; pb++) {
+        const long p = ((long)z * d.H + y0 + qy[pb]) * d.W + x0 + qx[pb];
+#pragma unroll
+        for (int ob = 0; ob < NOB; ob++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int o = ob * 32 + 8 * (v / 4) + 4 * (lane / 32) + v % 4;
+                if (o < d.Cout)
+                    y[((long)b * d.Cout + o) * P + p] = acc[pb][ob][v] + (bias ? bias[o] : 0.f);
+            }
+    }
+}
+
+}  // namespace gpd
+
+extern "C" int gpd_conv3d_forward_bf16(int B, int Cin, int Cout, int D, int H, int W,
+                                       const float* d_x, const float* d_w, const float* d_bias,
+                                       float* d_y, void* stream)
+{
+    GPD_REQUIRE(conv_dims_ok(B, Cin, Cout, D, H, W), GPD_ERR_ARG, "bad conv sizes");
+    GPD_REQUIRE(d_x && d_w && d_y && d_x != d_y, GPD_ERR_ARG, "bad arrays");
+    const int TX = W % 32 == 0 ? 32 : 16;
+    const int TY = 256 / TX;
+    if (Cin % kMCh != 0 || Cout > 64 || W % 16 != 0 || H % TY != 0) {
+        set_last_error("shape not supported by the bfloat16 MFMA convolution");
+        return GPD_ERR_UNSUPPORTED;
+    }
+    ConvDims d = {B, Cin, Cout, D, H, W};
+    const long nwork = (long)B * D * (H / TY) * (W / TX);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(xcd_grid_size(nwork)), block(256);
+    if (TX == 32) {
+        if (Cout <= 32)
+            hipLaunchKernelGGL((conv3d_fwd_mfma_kernel<32, 1>), grid, block, 0, st, d_x, d_w,
+                               d_bias, d_y, d);
+        else
+            hipLaunchKernelGGL((conv3d_fwd_mfma_kernel<32, 2>), grid, block, 0, st, d_x, d_w,
+                               d_bias, d_y, d);
+    } else {
+        if (Cout <= 32)
+            hipLaunchKernelGGL((conv3d_fwd_mfma_kernel<16, 1>), grid, block, 0, st, d_x, d_w,
+                               d_bias, d_y, d);
+        else
+            hipLaunchKernelGGL((conv3d_fwd_mfma_kernel<16, 2>), grid, block, 0, st, d_x, d_w,
+                               d_bias, d_y, d);
+    }
+    return last_error_to_status(hipGetLastError());
+}

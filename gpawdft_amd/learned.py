@@ -151,9 +151,11 @@ class UNet:
               'scale_add')
 
     # forward / backward -------------------------------------------------
-    def forward(self, x, keep=False):
+    def forward(self, x, keep=False, bf16=False):
         """x: (B, 2, D, H, W) FP32 on the device, D, H, W divisible by 4.
-        Returns (B, 1, D, H, W); keep=True stores what backward needs."""
+        Returns (B, 1, D, H, W); keep=True stores what backward needs.
+        bf16=True (inference) runs the convolutions whose shapes allow it
+        on MFMA with bfloat16 operands, the others in FP32."""
         assert x.ndim == 5 and x.shape[1] == 2
         assert all(s % 4 == 0 for s in x.shape[2:]), 'grid must divide by 4'
         x = x.contiguous()
@@ -167,10 +169,20 @@ class UNet:
                 B, _, D, H, W = h.shape
                 z = torch.empty((B, co, D, H, W), dtype=torch.float32,
                                 device=h.device)
-                check(lib().gpd_conv3d_forward(
-                    B, ci, co, D, H, W, _f32(h), _f32(self.param(name + '.w')),
-                    _f32(self.param(name + '.b')), _f32(z), 0, st),
-                    'conv3d_forward')
+                rc = -3
+                if bf16 and not keep:
+                    rc = lib().gpd_conv3d_forward_bf16(
+                        B, ci, co, D, H, W, _f32(h),
+                        _f32(self.param(name + '.w')),
+                        _f32(self.param(name + '.b')), _f32(z), st)
+                    if rc not in (0, -3):
+                        check(rc, 'conv3d_forward_bf16')
+                if rc != 0:
+                    check(lib().gpd_conv3d_forward(
+                        B, ci, co, D, H, W, _f32(h),
+                        _f32(self.param(name + '.w')),
+                        _f32(self.param(name + '.b')), _f32(z), 0, st),
+                        'conv3d_forward')
                 if name != last:
                     check(lib().gpd_leaky_relu(z.numel(), SLOPE, _f32(z),
                                                _f32(z), st), 'leaky_relu')
@@ -279,9 +291,10 @@ class LearnedPreconditioner:
 
     uses_eigenvalues = True
 
-    def __init__(self, mg, net, v_G):
+    def __init__(self, mg, net, v_G, bf16=True):
         self.mg = mg
         self.net = net
+        self.bf16 = bf16
         self.block = mg.block
         self.v = torch.as_tensor(np.asarray(v_G, np.float32)).to(
             net.device)
@@ -291,7 +304,7 @@ class LearnedPreconditioner:
 
     def learned_part(self, residuals, eps_x):
         x, s = network_input(residuals, self.v, eps_x)
-        y = self.net.forward(x)[:, 0]
+        y = self.net.forward(x, bf16=self.bf16)[:, 0]
         B = residuals.shape[0]
         return (y * s.view((B,) + (1,) * (residuals.ndim - 1))).to(
             residuals.dtype)

@@ -260,6 +260,14 @@ int gpd_conv3d_forward(int B, int Cin, int Cout, int D, int H, int W,
                        const float* d_x, const float* d_w,
                        const float* d_bias, float* d_y, int accumulate,
                        void* stream);
+/* Forward convolution for inference on MFMA with bfloat16 operands (inputs
+ * and weights rounded to nearest even, FP32 accumulation and output).
+ * Requires Cin % 8 == 0, Cout <= 64, W % 16 == 0 and H % (256 / TX) == 0
+ * with TX = 32 if 32 divides W, else 16; other shapes return
+ * GPD_ERR_UNSUPPORTED (use gpd_conv3d_forward). */
+int gpd_conv3d_forward_bf16(int B, int Cin, int Cout, int D, int H, int W,
+                            const float* d_x, const float* d_w,
+                            const float* d_bias, float* d_y, void* stream);
 int gpd_conv3d_backward_data(int B, int Cin, int Cout, int D, int H, int W,
                              const float* d_gy, const float* d_w,
                              float* d_gx, int accumulate, void* stream);

@@ -193,3 +193,64 @@ def test_training_beats_multigrid_on_held_out_systems(cuda):
     assert es_l.learned_offered == 5 * 16
     assert es_l.learned_kept > 0
     assert err_l[-1] <= 1.05 * err_m[-1], (err_l, err_m)
+
+
+def _bf16_round(a):
+    """Round to bfloat16, nearest even (the kernel's conversion)."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7fff + ((u >> 16) & 1)) & 0xffff0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+@pytest.mark.parametrize('B,Cin,Cout,shape', [(2, 16, 32, (4, 16, 64)),
+                                              (1, 32, 64, (3, 16, 16)),
+                                              (2, 8, 16, (2, 16, 48))])
+def test_conv3d_forward_bf16_mfma(cuda, B, Cin, Cout, shape):
+    """The MFMA convolution equals the exact convolution of the
+    bfloat16-rounded inputs and weights up to FP32 accumulation."""
+    import torch
+    from gpawdft_amd._lib import check, lib
+    from gpawdft_amd.device import stream_ptr
+    rng = np.random.default_rng(Cin * 7 + Cout)
+    x = rng.standard_normal((B, Cin) + shape).astype(np.float32)
+    w = (rng.standard_normal((Cout, Cin, 27)) / 5).astype(np.float32)
+    b = rng.standard_normal(Cout).astype(np.float32)
+    D, H, W = shape
+    y = torch.empty((B, Cout) + shape, dtype=torch.float32, device='cuda')
+    # keep the device copies referenced until the kernel has run
+    dx, dw, db = _dev(x), _dev(w), _dev(b)
+    check(lib().gpd_conv3d_forward_bf16(B, Cin, Cout, D, H, W,
+                                        dx.data_ptr(), dw.data_ptr(),
+                                        db.data_ptr(), y.data_ptr(),
+                                        stream_ptr()))
+    ref = U.conv3d(_bf16_round(x).astype(float), _bf16_round(w).astype(float),
+                   b.astype(float))
+    assert _rel(_np(y), ref) < 1e-5
+    # and it stays close to the FP32 convolution (bfloat16 inputs)
+    assert _rel(_np(y), U.conv3d(x.astype(float), w.astype(float),
+                                 b.astype(float))) < 2e-2
+
+
+def test_conv3d_forward_bf16_rejects_unsupported_shapes(cuda):
+    from gpawdft_amd._lib import lib
+    import torch
+    x = torch.zeros((1, 8, 4, 8, 24), device='cuda')
+    w = torch.zeros((16, 8, 27), device='cuda')
+    y = torch.zeros((1, 16, 4, 8, 24), device='cuda')
+    assert lib().gpd_conv3d_forward_bf16(1, 8, 16, 4, 8, 24, x.data_ptr(),
+                                         w.data_ptr(), None, y.data_ptr(),
+                                         None) == -3
+
+
+def test_network_inference_in_bf16_matches_fp32(cuda):
+    """The trained network in bfloat16 MFMA inference gives the same
+    correction as in FP32 to bfloat16 accuracy."""
+    import os
+    import torch
+    from gpawdft_amd.learned import UNet
+    net = UNet.load(os.path.join(os.path.dirname(__file__), '..',
+                                 'gpawdft_amd', 'data', 'unet_precond.npz'))
+    x = torch.randn((2, 2, 32, 32, 64), device='cuda')
+    y32 = net.forward(x)
+    y16 = net.forward(x, bf16=True)
+    assert float((y16 - y32).norm() / y32.norm()) < 2e-2

@@ -34,60 +34,62 @@ constexpr int kNF = 16;   // functions per group held in registers
 // ------------------------------------------------------------------ //
 // kernels
 
-// FP64-exact integrate: one thread per (band, function block), bands
-// fastest so that a wave shares the block's point list (scalar loads).
-// Points are taken in batches of 8 with all loads issued first; the adds
-// into c stay in the reference's sequence.
+// FP64-exact integrate: one thread per (band, function M), bands fastest,
+// so the lanes of a wave walk one function's point list together (the
+// index and A loads are the same for all of them).  Each chain adds in the
+// reference's order; the loads of the next batch of points are issued
+// before the adds of the current one.
 template <typename T>
-__global__ void lfc_integrate_seq_kernel(const T* __restrict__ a, T* __restrict__ c, long nG,
-                                         int Mmax, int nkeys, long nx, T dv,
-                                         const T* __restrict__ A,
-                                         const int* __restrict__ key_M,
-                                         const int* __restrict__ key_nm,
-                                         const long* __restrict__ key_off,
-                                         const int* __restrict__ key_G,
-                                         const int* __restrict__ key_row)
+__global__ __launch_bounds__(64) void lfc_integrate_seq_kernel(
+    const T* __restrict__ a, T* __restrict__ c, long nG, int Mmax, long nx, T dv,
+    const T* __restrict__ A, const int* __restrict__ fun_key, const int* __restrict__ key_M,
+    const int* __restrict__ key_nm, const long* __restrict__ key_off,
+    const int* __restrict__ key_G, const int* __restrict__ key_row)
 {
-    constexpr int BATCH = 8;
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nx * nkeys) return;
+    constexpr int BATCH = 16;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * Mmax) return;
     const long band = idx % nx;
-    const int k = (int)(idx / nx);
-    const int M = key_M[k], nm = key_nm[k];
+    const int Mi = (int)(idx / nx);
+    const int k = fun_key[Mi];
+    if (k < 0) return;
+    const int m = Mi - key_M[k];
+    const int nm = key_nm[k];
     const T* ab = a + band * nG;
-    T* cb = c + band * Mmax + M;
-    T acc[kNF];
-#pragma unroll
-    for (int m = 0; m < kNF; m++) acc[m] = m < nm ? cb[m] : T(0);
+    T* cp = c + band * Mmax + Mi;
+    T acc = *cp;
     const long j0 = key_off[k], j1 = key_off[k + 1];
+
+    T xa[BATCH], Aa[BATCH], xb[BATCH], Ab[BATCH];
+    auto load = [&](long j, T* xv, T* Av) {
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) {
+            if (j + q < j1) {
+                xv[q] = ab[key_G[j + q]];
+                Av[q] = A[(long)key_row[j + q] + m];
+            }
+        }
+    };
+    auto add = [&](long j, const T* xv, const T* Av) {
+#pragma unroll
+        for (int q = 0; q < BATCH; q++)
+            if (j + q < j1) acc = madd<T>(acc, xv[q] * dv, Av[q]);
+    };
+    (void)nm;
     long j = j0;
-    for (; j + BATCH <= j1; j += BATCH) {
-        T av[BATCH];
-        int row[BATCH];
-#pragma unroll
-        for (int q = 0; q < BATCH; q++) {
-            av[q] = ab[key_G[j + q]];
-            row[q] = key_row[j + q];
-        }
-#pragma unroll
-        for (int q = 0; q < BATCH; q++) {
-            const T x = av[q] * dv;
-            const T* Ar = A + row[q];
-#pragma unroll
-            for (int m = 0; m < kNF; m++)
-                if (m < nm) acc[m] = madd<T>(acc[m], x, Ar[m]);
-        }
+    if (j < j1) load(j, xa, Aa);
+    while (j < j1) {
+        const long jn = j + BATCH;
+        if (jn < j1) load(jn, xb, Ab);
+        add(j, xa, Aa);
+        j = jn;
+        if (j >= j1) break;
+        const long jm = j + BATCH;
+        if (jm < j1) load(jm, xa, Aa);
+        add(j, xb, Ab);
+        j = jm;
     }
-    for (; j < j1; j++) {
-        const T x = ab[key_G[j]] * dv;
-        const T* Ar = A + key_row[j];
-#pragma unroll
-        for (int m = 0; m < kNF; m++)
-            if (m < nm) acc[m] = madd<T>(acc[m], x, Ar[m]);
-    }
-#pragma unroll
-    for (int m = 0; m < kNF; m++)
-        if (m < nm) cb[m] = acc[m];
+    *cp = acc;
 }
 
 // Grouped integrate: one workgroup per (group, block of NBB bands).  NF is
@@ -212,29 +214,42 @@ __global__ __launch_bounds__(256) void lfc_add_grp_kernel(
     }
 }
 
-// add: one thread per (band, covered point), entries in sweep order.
-template <typename T>
+// add: one thread per (covered point, NBT bands), entries in sweep order.
+// Each A value is loaded once for the NBT bands; every band's sum keeps
+// the reference's order.
+template <typename T, int NBT>
 __global__ void lfc_add_kernel(const T* __restrict__ c, T* __restrict__ a, long nG, int Mmax,
                                long nx, long ncov, const T* __restrict__ A,
                                const int* __restrict__ cov_G, const int* __restrict__ cov_ptr,
                                const int2* __restrict__ ent)
 {
+    const long nbblk = (nx + NBT - 1) / NBT;
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nx * ncov) return;
+    if (idx >= nbblk * ncov) return;
     const long i = idx % ncov;
-    const long band = idx / ncov;
-    T* ap = a + band * nG + cov_G[i];
-    const T* cb = c + band * Mmax;
-    T v = *ap;
+    const long b0 = (idx / ncov) * NBT;
+    const long G = cov_G[i];
+    T v[NBT];
+#pragma unroll
+    for (int bb = 0; bb < NBT; bb++) v[bb] = b0 + bb < nx ? a[(b0 + bb) * nG + G] : T(0);
     const int e1 = cov_ptr[i + 1];
     for (int e = cov_ptr[i]; e < e1; e++) {
         const int2 en = ent[e];
         const int nm = en.y & 15;
         const T* Ar = A + en.x;
-        const T* cm = cb + (en.y >> 4);
-        for (int m = 0; m < nm; m++) v = madd<T>(v, Ar[m], cm[m]);
+        const int M = en.y >> 4;
+        for (int m = 0; m < nm; m++) {
+            const T Am = Ar[m];
+#pragma unroll
+            for (int bb = 0; bb < NBT; bb++) {
+                const long b = b0 + bb < nx ? b0 + bb : b0;
+                v[bb] = madd<T>(v[bb], Am, c[b * Mmax + M + m]);
+            }
+        }
     }
-    *ap = v;
+#pragma unroll
+    for (int bb = 0; bb < NBT; bb++)
+        if (b0 + bb < nx) a[(b0 + bb) * nG + G] = v[bb];
 }
 
 // ------------------------------------------------------------------ //
@@ -358,6 +373,16 @@ static hipError_t upload(V** dst, const V* src, size_t n)
     if (e != hipSuccess) return e;
     if (src && n) return hipMemcpy(*dst, src, sizeof(V) * n, hipMemcpyHostToDevice);
     return hipMemset(*dst, 0, sizeof(V) * (n ? n : 1));
+}
+
+int lfc_add_exact_f64(const gpd_lfc* l, long nx, const double* c, double* a, hipStream_t st)
+{
+    if (nx == 0 || l->ncov == 0) return GPD_OK;
+    constexpr int NBT = 8;
+    const long n = ((nx + NBT - 1) / NBT) * l->ncov;
+    hipLaunchKernelGGL((lfc_add_kernel<double, NBT>), dim3((n + 255) / 256), dim3(256), 0, st, c,
+                       a, l->nG, l->Mmax, nx, l->ncov, l->A64, l->cov_G, l->cov_ptr, l->ent);
+    return last_error_to_status(hipGetLastError());
 }
 
 template <int NB, int NF>
@@ -505,7 +530,7 @@ extern "C" int gpd_lfc_destroy(gpd_lfc* l)
 {
     if (!l) return GPD_OK;
     void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
-                    l->key_off, l->key_G, l->key_row};
+                    l->key_off, l->key_G, l->key_row, l->fun_key};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_groups(l->gi);
@@ -543,6 +568,11 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     l->ncov = (long)h.cov_G.size();
     l->nent = (long)h.ent.size();
     l->nkeys = (int)h.key_M.size();
+
+    // function M -> its key (-1 for an index no volume carries)
+    std::vector<int> fkey(std::max(Mmax, 1), -1);
+    for (int k = 0; k < l->nkeys; k++)
+        for (int m = 0; m < h.key_nm[k]; m++) fkey[h.key_M[k] + m] = k;
 
     // per-key flat arrays
     std::vector<long> key_off(l->nkeys + 1, 0);
@@ -604,6 +634,7 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     UP(key_off, key_off.data(), key_off.size());
     UP(key_G, kG.data(), kG.size());
     UP(key_row, kR.data(), kR.size());
+    UP(fun_key, fkey.data(), fkey.size());
 #undef UP
     if (e != hipSuccess) {
         set_last_error(hipGetErrorString(e));
@@ -623,10 +654,11 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
     GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
     hipStream_t st = (hipStream_t)stream;
     if (dtype == GPD_F64) {
-        long n = nx * l->nkeys;
+        long n = nx * l->Mmax;
         hipLaunchKernelGGL(lfc_integrate_seq_kernel<double>, dim3((n + 63) / 64), dim3(64), 0, st,
-                           (const double*)d_a_xG, (double*)d_c_xM, l->nG, l->Mmax, l->nkeys, nx,
-                           l->dv, l->A64, l->key_M, l->key_nm, l->key_off, l->key_G, l->key_row);
+                           (const double*)d_a_xG, (double*)d_c_xM, l->nG, l->Mmax, nx, l->dv,
+                           l->A64, l->fun_key, l->key_M, l->key_nm, l->key_off, l->key_G,
+                           l->key_row);
         return last_error_to_status(hipGetLastError());
     }
     if (dtype == GPD_F32) {
@@ -661,13 +693,7 @@ extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, v
     if (nx == 0 || l->ncov == 0) return GPD_OK;
     GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
     hipStream_t st = (hipStream_t)stream;
-    long n = nx * l->ncov;
-    if (dtype == GPD_F64) {
-        hipLaunchKernelGGL(lfc_add_kernel<double>, dim3((n + 255) / 256), dim3(256), 0, st,
-                           (const double*)d_c_xM, (double*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
-                           l->A64, l->cov_G, l->cov_ptr, l->ent);
-        return last_error_to_status(hipGetLastError());
-    }
+    if (dtype == GPD_F64) return lfc_add_exact_f64(l, nx, (const double*)d_c_xM, (double*)d_a_xG, st);
     if (dtype == GPD_F32)
         return lfc_add_grouped_f32(l, nx, (const float*)d_c_xM, (float*)d_a_xG, st);
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");

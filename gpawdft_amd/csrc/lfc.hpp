@@ -61,6 +61,7 @@ struct gpd_lfc {
     long* key_off = nullptr;   // nkeys + 1 into key_G / key_row
     int* key_G = nullptr;
     int* key_row = nullptr;
+    int* fun_key = nullptr;    // Mmax: key of each function index
     LfcGroups gi;              // grouped integrate (FP32), from the keys
     LfcGroups ga;              // grouped add (FP32), from the sub-keys
     bool add_shares_groups = true;   // no repeated points: ga aliases gi
@@ -70,4 +71,6 @@ namespace gpd {
 template <typename T> const T* lfc_values(const gpd_lfc* l);
 // FP32 add by colours of disjoint groups (fast path, order not preserved)
 int lfc_add_grouped_f32(const gpd_lfc* l, long nx, const float* c, float* a, hipStream_t st);
+// FP64 add in the reference's per-point order (bit-compatible)
+int lfc_add_exact_f64(const gpd_lfc* l, long nx, const double* c, double* a, hipStream_t st);
 }

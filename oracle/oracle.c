@@ -29,6 +29,14 @@
  *                                reference calls BLAS daxpy there, whose FMA
  *                                use depends on the BLAS build: that one step
  *                                is pinned to 1 ulp, not bitwise.
+ *   orc_add_local_potential   -- the reference's numpy expression
+ *                                Htpsit += psit * vt, bit-for-bit.
+ *   orc_projector_coefs       -- numpy P @ dH with a non-symmetric dH, to
+ *                                1e-14 (the reference's np.dot goes to BLAS,
+ *                                whose summation order is not pinned).
+ *
+ * The RMM-DIIS restatement (rmmdiis.py) and the learned network
+ * (unet.py) are pinned in their own headers.
  */
 #include <math.h>
 #include <stdint.h>

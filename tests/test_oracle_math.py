@@ -262,3 +262,34 @@ def test_rmmdiis_restatement_converges_to_dense_spectrum():
     assert np.abs(eps[:6] - w[:6]).max() < 1e-12
     S = sysm.gd.dv * psi @ psi.T
     assert np.abs(S - np.eye(8)).max() < 1e-12
+
+
+def test_local_potential_matches_the_reference_expression():
+    """orc_add_local_potential is bit-for-bit the reference's numpy
+    expression `Htpsit_G += psit_G * vt_G` (gpaw/hamiltonian.py:419-424):
+    one rounded product, one rounded sum."""
+    rng = np.random.default_rng(41)
+    psi = rng.standard_normal((3, 6, 5, 4))
+    v = rng.standard_normal((6, 5, 4))
+    h = rng.standard_normal((3, 6, 5, 4))
+    ref = h.copy()
+    for n in range(3):
+        ref[n] += psi[n] * v
+    got = O.add_local_potential(psi, v, h)
+    assert np.array_equal(got.view(np.int64), ref.view(np.int64))
+
+
+def test_projector_coefs_is_P_times_dH():
+    """c_a = P_a dH_a (gpaw/hamiltonian.py:456-458, np.dot(P_xi, dH_ii)),
+    checked against numpy with a non-symmetric dH so that a transposed
+    operand fails; the reference's BLAS order is not pinned, so to 1e-14."""
+    rng = np.random.default_rng(42)
+    P = rng.standard_normal((5, 14))
+    dH0 = rng.standard_normal((9, 9))
+    dH1 = rng.standard_normal((5, 5))
+    c = O.projector_coefs(P, [(0, dH0), (9, dH1)])
+    ref = np.zeros_like(P)
+    ref[:, 0:9] = P[:, 0:9] @ dH0
+    ref[:, 9:14] = P[:, 9:14] @ dH1
+    assert np.abs(c - ref).max() < 1e-14 * np.abs(ref).max()
+    assert np.abs(c[:, :9] - P[:, :9] @ dH0.T).max() > 1e-3

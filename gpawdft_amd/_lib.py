@@ -37,6 +37,7 @@ _SIGNATURES = {
                                  _c_long_p, _i, _i, _vpp]),
     'gpd_operator_apply': (_i, [_vp, _i, _l, _vp, _vp, _vp, _vp]),
     'gpd_operator_set_launch': (_i, [_vp, _l]),
+    'gpd_operator_relax': (_i, [_vp, _i, _i, _l, _vp, _vp, _i, _d, _vp]),
     'gpd_operator_get_diagonal_element': (_i, [_vp, _c_double_p]),
     'gpd_operator_destroy': (_i, [_vp]),
     'gpd_transformer_create': (_i, [_c_long_p, _c_long_p, _i, _c_long_p,

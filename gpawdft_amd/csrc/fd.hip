@@ -277,6 +277,44 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
 }
 
 // ------------------------------------------------------------------ //
+// One weighted Jacobi sweep of Operator.relax (c/bmgs/relax.c:46-72):
+// x = sum_{c >= 1} coef_c f[p + off_c] (from 0, in coefficient order,
+// coefficient 0 being the centre), then
+// out = (1 - w) f + (w (src - x)) / coef_0, every neighbour from the
+// previous sweep's f (periodic or zero outside).
+template <typename T>
+__global__ void fd_jacobi_kernel(const T* __restrict__ f, const T* __restrict__ src,
+                                 T* __restrict__ out, int n0, int n1, int n2, int pbc0, int pbc1,
+                                 int pbc2, int ncoefs, const int* __restrict__ off,
+                                 const T* __restrict__ coefs, T w)
+{
+    const long nG = (long)n0 * n1 * n2;
+    const long band = blockIdx.y;
+    const T* a = f + band * nG;
+    const T* sr = src + band * nG;
+    T* b = out + band * nG;
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nG;
+         p += (long)gridDim.x * blockDim.x) {
+        const int i2 = (int)(p % n2);
+        const long t = p / n2;
+        const int i1 = (int)(t % n1);
+        const int i0 = (int)(t / n1);
+        T x = T(0);
+        for (int c = 1; c < ncoefs; c++) {
+            const int j0 = wrap_index(i0 + off[3 * c], n0, pbc0);
+            const int j1 = wrap_index(i1 + off[3 * c + 1], n1, pbc1);
+            const int j2 = wrap_index(i2 + off[3 * c + 2], n2, pbc2);
+            T v = T(0);
+            if (j0 >= 0 && j1 >= 0 && j2 >= 0) v = a[((long)j0 * n1 + j1) * n2 + j2];
+            x = madd<T>(x, v, coefs[c]);
+        }
+        const T t1 = (T(1) - w) * a[p];
+        const T t2 = (w * (sr[p] - x)) / coefs[0];
+        b[p] = t1 + t2;
+    }
+}
+
+// ------------------------------------------------------------------ //
 // Epilogue for stencils without the axis form: s was computed by the
 // generic kernel into `s_buf`; same per-point arithmetic as the axis kernel.
 template <typename T, int EPI>
@@ -623,6 +661,68 @@ extern "C" int gpd_operator_set_launch(gpd_operator* op, long wg_target)
     GPD_REQUIRE(wg_target >= 1 && wg_target <= (1L << 24), GPD_ERR_ARG, "bad workgroup target");
     op->wg_target = wg_target;
     return GPD_OK;
+}
+
+template <typename T>
+static int relax_jacobi(const gpd_operator* op, long nin, T* fun, const T* src, int nrelax, T w,
+                        hipStream_t st)
+{
+    const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+    T* tmp = nullptr;
+    hipError_t e = hipMallocAsync((void**)&tmp, sizeof(T) * nin * nG, st);
+    if (e != hipSuccess) {
+        set_last_error(hipGetErrorString(e));
+        return GPD_ERR_HIP;
+    }
+    const T* coefs = std::is_same<T, double>::value ? (const T*)op->d_coefs64
+                                                    : (const T*)op->d_coefs32;
+    const long blocks = std::min<long>((nG + 255) / 256, 4096);
+    int rc = GPD_OK;
+    T* cur = fun;
+    T* nxt = tmp;
+    for (int it = 0; it < nrelax && rc == GPD_OK; it++) {
+        for (long b0 = 0; b0 < nin && rc == GPD_OK; b0 += 65535) {
+            const long nbb = std::min<long>(65535, nin - b0);
+            hipLaunchKernelGGL(fd_jacobi_kernel<T>, dim3(blocks, nbb), dim3(256), 0, st,
+                               cur + b0 * nG, src + b0 * nG, nxt + b0 * nG, op->n[0], op->n[1],
+                               op->n[2], op->pbc[0], op->pbc[1], op->pbc[2], op->ncoefs,
+                               op->d_off, coefs, w);
+            rc = last_error_to_status(hipGetLastError());
+        }
+        std::swap(cur, nxt);
+    }
+    if (rc == GPD_OK && cur != fun) {
+        e = hipMemcpyAsync(fun, cur, sizeof(T) * nin * nG, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            rc = GPD_ERR_HIP;
+        }
+    }
+    (void)hipFreeAsync(tmp, st);
+    return rc;
+}
+
+extern "C" int gpd_operator_relax(gpd_operator* op, int dtype, int relax_method, long nin,
+                                  void* d_fun, const void* d_src, int nrelax, double w,
+                                  void* stream)
+{
+    GPD_REQUIRE(op, GPD_ERR_ARG, "null operator");
+    GPD_REQUIRE(nin >= 0 && nrelax >= 0, GPD_ERR_ARG, "negative count");
+    GPD_REQUIRE(relax_method == 1 || relax_method == 2, GPD_ERR_ARG, "relax_method is 1 or 2");
+    GPD_REQUIRE(relax_method == 2, GPD_ERR_UNSUPPORTED,
+                "Gauss-Seidel relaxation (relax_method 1) is a serial lexicographic sweep; "
+                "only weighted Jacobi (2) runs on the GPU");
+    GPD_REQUIRE(op->ncoefs >= 1 && op->off[0] == 0 && op->off[1] == 0 && op->off[2] == 0,
+                GPD_ERR_UNSUPPORTED, "relax needs the centre as coefficient 0");
+    if (nin == 0 || nrelax == 0) return GPD_OK;
+    GPD_REQUIRE(d_fun && d_src && d_fun != d_src, GPD_ERR_ARG, "bad arrays");
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_F64)
+        return relax_jacobi<double>(op, nin, (double*)d_fun, (const double*)d_src, nrelax, w, st);
+    if (dtype == GPD_F32)
+        return relax_jacobi<float>(op, nin, (float*)d_fun, (const float*)d_src, nrelax, (float)w,
+                                   st);
+    GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
 }
 
 extern "C" int gpd_operator_get_diagonal_element(const gpd_operator* op, double* diag)

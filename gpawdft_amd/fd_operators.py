@@ -108,8 +108,24 @@ class FDOperator:
               'Operator.set_launch')
 
     def relax(self, relax_method, f_g, s_g, n, w=None):
-        raise NotImplementedError('relaxation (Poisson solver) is not part '
-                                  'of this build')
+        """n relaxation sweeps of f_g in place for "operator f = s"
+        (gpaw/fd_operators.py:78-80, c/operators.c:49-91).  relax_method 2
+        is weighted Jacobi with weight w (1.0 if None); Gauss-Seidel (1)
+        raises NotImplementedError."""
+        if tuple(f_g.shape) != tuple(s_g.shape):
+            raise ValueError('function and source shapes differ')
+        if tuple(f_g.shape[-3:]) != self.shape:
+            raise ValueError('grid shape does not match the operator')
+        vf = DeviceView(f_g)
+        vs = DeviceView(s_g)
+        if vf.code != vs.code:
+            raise TypeError('function and source dtypes differ')
+        nin = int(np.prod(f_g.shape[:-3], dtype=np.int64))
+        check(lib().gpd_operator_relax(self._handle, vf.code, int(relax_method),
+                                       nin, vf.ptr, vs.ptr, int(n),
+                                       1.0 if w is None else float(w),
+                                       stream_ptr()), 'Operator.relax')
+        vf.finish()
 
     def get_diagonal_element(self):
         d = _lib.ctypes.c_double()
@@ -201,3 +217,48 @@ class GUCLaplace(FDOperator):
         self.description = (
             '%d*%d+1=%d point O(h^%d) finite-difference Laplacian' %
             ((self.npoints - 1) // n, n, self.npoints, 2 * n))
+
+
+def mehrstellen_stencils(gd, scale):
+    """Coefficients and offsets of the O(h^4) Mehrstellen pair
+    (gpaw/fd_operators.py:211-251) on an orthogonal cell.  A: centre, the
+    six face neighbours, then the twelve edge neighbours in the yz, xz and
+    xy planes; B: 1/2 at the centre, 1/12 at the six faces.  Used as
+    A phi = B rho."""
+    if not gd.orthogonal:
+        raise ValueError('the Mehrstellen stencil needs an orthogonal cell')
+    # per axis; the same operation order as the reference (bit-compatible)
+    edge = np.divide(-1 / 12, gd.h_cv.diagonal()**2) * scale
+    e_yz = edge[1] + edge[2]
+    e_xz = edge[0] + edge[2]
+    e_xy = edge[0] + edge[1]
+    centre = -16.0 * edge.sum()
+    face = 10.0 * edge + 0.125 * centre
+    coefs_a = ([centre, face[0], face[0], face[1], face[1], face[2], face[2]]
+               + [e_yz] * 4 + [e_xz] * 4 + [e_xy] * 4)
+    offs_a = [(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0),
+              (0, 0, -1), (0, 0, 1),
+              (0, -1, -1), (0, -1, 1), (0, 1, -1), (0, 1, 1),
+              (-1, 0, -1), (-1, 0, 1), (1, 0, -1), (1, 0, 1),
+              (-1, -1, 0), (-1, 1, 0), (1, -1, 0), (1, 1, 0)]
+    coefs_b = [0.5] + [1.0 / 12.0] * 6
+    offs_b = offs_a[:7]
+    return (coefs_a, offs_a), (coefs_b, offs_b)
+
+
+class LaplaceA(FDOperator):
+    """O(h^4) Mehrstellen Laplacian, left-hand side (19 points)."""
+
+    def __init__(self, gd, scale, dtype=float):
+        (c, o), _ = mehrstellen_stencils(gd, scale)
+        FDOperator.__init__(self, c, o, gd, dtype,
+                            'O(h^4) Mehrstellen Laplacian (A)')
+
+
+class LaplaceB(FDOperator):
+    """O(h^4) Mehrstellen Laplacian, right-hand side (7 points)."""
+
+    def __init__(self, gd, dtype=float):
+        _, (c, o) = mehrstellen_stencils(gd, 1.0)
+        FDOperator.__init__(self, c, o, gd, dtype,
+                            'O(h^4) Mehrstellen Laplacian (B)')

@@ -1,0 +1,201 @@
+"""Finite-difference multigrid Poisson solver on the device.
+
+Drop-in for gpaw/poisson.py:47-315: `PoissonSolver(nn=3, relax='J',
+eps=2e-10, maxiter=1000)` with `set_grid_descriptor(gd)`, `initialize()`,
+`solve(phi, rho, charge=None, eps=None, maxcharge=1e-6,
+zero_initial_phi=False)` returning the iteration count, and `iterate2`.
+The operator is -1/(4 pi) times the nn-th order Laplacian on the fine grid
+(or the Mehrstellen pair A phi = B rho for nn='M'), 7-point Laplacians on
+the coarse grids, nn=1 transformers between levels, weighted Jacobi
+smoothing (2 pre- / 1 post-sweeps at weight 2/3 on the fine grid, 4 / 4 at
+weight 1 below, 8 / 8 on the coarsest).  Every step runs in this library's
+kernels: relax, stencil, restrict / interpolate, batched axpy and dot.
+
+Not covered: Gauss-Seidel smoothing (a serial lexicographic sweep),
+charged systems with zero boundaries (they need gpaw.utilities.gauss), the
+multipole removal and the FFT / dipole-layer solvers.
+"""
+from math import pi
+
+import numpy as np
+
+from gpawdft_amd.device import DeviceView, torch
+from gpawdft_amd.eigensolvers import axpy, dots
+from gpawdft_amd.fd_operators import Laplace, LaplaceA, LaplaceB
+from gpawdft_amd.transformers import Transformer
+
+
+class PoissonConvergenceError(RuntimeError):
+    pass
+
+
+def create_poisson_solver(name='fd', **kwargs):
+    if name == 'fd':
+        return PoissonSolver(**kwargs)
+    raise NotImplementedError('Poisson solver %r is not part of this build'
+                              % name)
+
+
+def PoissonSolver(dipolelayer=None, **kwargs):
+    if dipolelayer is not None:
+        raise NotImplementedError('dipole-layer correction is not part of '
+                                  'this build')
+    return FDPoissonSolver(**kwargs)
+
+
+class FDPoissonSolver:
+    def __init__(self, nn=3, relax='J', eps=2e-10, maxiter=1000,
+                 remove_moment=None, use_charge_center=False):
+        if relax == 'GS':
+            raise NotImplementedError(
+                'Gauss-Seidel smoothing is a serial sweep; use relax="J"')
+        if relax != 'J':
+            raise NotImplementedError('Relaxation method %s' % relax)
+        if remove_moment is not None:
+            raise NotImplementedError('multipole removal is not part of '
+                                      'this build')
+        self.relax = relax
+        self.relax_method = 2
+        self.nn = nn
+        self.eps = eps
+        self.maxiter = maxiter
+        self.remove_moment = remove_moment
+        self.use_charge_center = use_charge_center
+        self.description = None
+
+    def todict(self):
+        return {'name': 'fd', 'nn': self.nn, 'relax': self.relax,
+                'eps': self.eps, 'remove_moment': self.remove_moment}
+
+    def get_stencil(self):
+        return self.nn
+
+    def set_grid_descriptor(self, gd):
+        self.gd = gd
+        scale = -0.25 / pi
+        if self.nn == 'M':
+            self.operators = [LaplaceA(gd, -scale)]
+            self.B = LaplaceB(gd)
+        else:
+            self.operators = [Laplace(gd, scale, self.nn)]
+            self.B = None
+        self.interpolators = []
+        self.restrictors = []
+        self.presmooths = [2]
+        self.postsmooths = [1]
+        self.weights = [2.0 / 3.0]
+        level = 0
+        while level < 8:
+            try:
+                gd2 = gd.coarsen()
+            except ValueError:
+                break
+            self.operators.append(Laplace(gd2, scale, 1))
+            self.interpolators.append(Transformer(gd2, gd))
+            self.restrictors.append(Transformer(gd, gd2))
+            self.presmooths.append(4)
+            self.postsmooths.append(4)
+            self.weights.append(1.0)
+            level += 1
+            gd = gd2
+        self.levels = level
+
+    def get_description(self):
+        coarsest = ' x '.join(str(N) for N in self.operators[-1].gd.N_c)
+        return ('Jacobi solver with %d multi-grid levels\n'
+                '    Coarsest grid: %s points\n    Stencil: %s\n'
+                '    Tolerance: %e\n    Max iterations: %d'
+                % (self.levels + 1, coarsest,
+                   self.operators[0].description, self.eps, self.maxiter))
+
+    def initialize(self, load_gauss=False, device='cuda'):
+        gd = self.gd
+        dt = torch.float64
+        self.rhos = [gd.empty(device=device)]
+        self.phis = [None]
+        self.residuals = [gd.empty(device=device)]
+        for level in range(self.levels):
+            gd2 = gd.coarsen()
+            self.phis.append(gd2.empty(device=device))
+            self.rhos.append(gd2.empty(device=device))
+            self.residuals.append(gd2.empty(device=device))
+            gd = gd2
+        self.step = 0.66666666 / self.operators[0].get_diagonal_element()
+        self.presmooths[self.levels] = 8
+        self.postsmooths[self.levels] = 8
+        self._ones = torch.ones(tuple(int(n) for n in self.gd.n_c), dtype=dt,
+                                device=device)
+        if load_gauss:
+            raise NotImplementedError('Gaussian compensation charges are '
+                                      'not part of this build')
+
+    def _sum(self, a):
+        """sum of a grid (a batched dot with ones)."""
+        return float(dots(a[None], self._ones[None])[0])
+
+    def solve(self, phi, rho, charge=None, eps=None, maxcharge=1e-6,
+              zero_initial_phi=False):
+        """phi (in/out) and rho: FP64 device tensors or numpy arrays of the
+        grid's shape.  Returns the number of V-cycles."""
+        assert tuple(phi.shape) == tuple(self.gd.n_c)
+        assert tuple(rho.shape) == tuple(self.gd.n_c)
+        if eps is None:
+            eps = self.eps
+        vphi = DeviceView(phi)
+        vrho = DeviceView(rho)
+        actual_charge = self._sum(vrho.t) * self.gd.dv
+        background = actual_charge / self.gd.dv / np.prod(self.gd.N_c)
+        if charge is None:
+            charge = actual_charge
+        if abs(charge) > maxcharge and not self.gd.pbc_c.all():
+            raise NotImplementedError('charged systems need periodic '
+                                      'boundaries in this build')
+        if abs(charge) > maxcharge and zero_initial_phi:
+            vphi.t.zero_()
+        neutral = vrho.t.clone()
+        axpy(np.array([-background]), self._ones[None], neutral[None])
+        niter = self.solve_neutral(vphi.t, neutral, eps=eps)
+        vphi.finish()
+        return niter
+
+    def solve_neutral(self, phi, rho, eps=2e-10):
+        self.phis[0] = phi
+        if self.B is None:
+            self.rhos[0].copy_(rho)
+        else:
+            self.B.apply(rho, self.rhos[0])
+        niter = 1
+        while self.iterate2(self.step) > eps and niter < self.maxiter:
+            niter += 1
+        if niter == self.maxiter:
+            raise PoissonConvergenceError(
+                'Poisson solver did not converge in %d iterations!'
+                % self.maxiter)
+        # zero average potential on periodic grids
+        if self.gd.pbc_c.all():
+            ave = self._sum(phi) / np.prod(self.gd.N_c)
+            axpy(np.array([-ave]), self._ones[None], phi[None])
+        return niter
+
+    def iterate2(self, step, level=0):
+        """One V-cycle from `level` (gpaw/poisson.py:278-311); at level 0
+        returns sum(residual^2) dv."""
+        residual = self.residuals[level]
+        op = self.operators[level]
+        if level < self.levels:
+            op.relax(self.relax_method, self.phis[level], self.rhos[level],
+                     self.presmooths[level], self.weights[level])
+            op.apply(self.phis[level], residual)
+            axpy(np.array([-1.0]), self.rhos[level][None], residual[None])
+            self.restrictors[level].apply(residual, self.rhos[level + 1])
+            self.phis[level + 1].zero_()
+            self.iterate2(4.0 * step, level + 1)
+            self.interpolators[level].apply(self.phis[level + 1], residual)
+            axpy(np.array([-1.0]), residual[None], self.phis[level][None])
+        op.relax(self.relax_method, self.phis[level], self.rhos[level],
+                 self.postsmooths[level], self.weights[level])
+        if level == 0:
+            op.apply(self.phis[level], residual)
+            axpy(np.array([-1.0]), self.rhos[level][None], residual[None])
+            return float(dots(residual[None], residual[None],
+                              self.gd.dv)[0])

@@ -29,6 +29,8 @@
  *                                reference calls BLAS daxpy there, whose FMA
  *                                use depends on the BLAS build: that one step
  *                                is pinned to 1 ulp, not bitwise.
+ *   orc_relax                 -- reference Operator.relax (oracle/_ref)
+ *                                bit-for-bit, both methods (golden digests).
  *   orc_add_local_potential   -- the reference's numpy expression
  *                                Htpsit += psit * vt, bit-for-bit.
  *   orc_projector_coefs       -- numpy P @ dH with a non-symmetric dH, to
@@ -46,6 +48,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
 
 /* ------------------------------------------------------------------ */
 /* Halo fill.  Follows c/bc.c:17-135 (bc_init) and c/bc.c:138-250
@@ -153,6 +156,67 @@ void orc_fd(int ncoefs, const double* coefs, const long* off_c,
         }
         free(buf);
     }
+    free(flat);
+}
+
+/* ------------------------------------------------------------------ */
+/* Operator.relax (c/operators.c:49-91) with bmgs_relax (c/bmgs/relax.c:
+ * 7-73).  Every sweep fills the halo from fun (c/bc.c, as orc_halo_fill),
+ * then walks the interior in lexicographic order with the stencil minus
+ * its centre (coefficient 0), summed from 0.0 in coefficient order:
+ *   method 1 (Gauss-Seidel): x = (src - x) * (1 / coef_0), written to fun
+ *     and into the padded buffer, so later points see it;
+ *   method 2 (weighted Jacobi): fun = (1 - w) fun + w (src - x) / coef_0,
+ *     all neighbours from the buffer of the previous sweep.            */
+void orc_relax(int method, int ncoefs, const double* coefs, const long* off_c,
+               const long n[3], const int pbc[3], long nb, double* fun,
+               const double* src, int nrelax, double w)
+{
+    long r = 0;
+    int corners = 0;
+    for (int c = 0; c < ncoefs; c++) {
+        int nonzero = 0;
+        for (int k = 0; k < 3; k++) {
+            long d = off_c[3 * c + k];
+            if (d > r) r = d;
+            if (d != 0) nonzero++;
+        }
+        if (nonzero >= 2) corners = 1;
+    }
+    long lo[3] = {r, r, r}, hi[3] = {r, r, r};
+    long m[3] = {n[0] + 2 * r, n[1] + 2 * r, n[2] + 2 * r};
+    long ng = n[0] * n[1] * n[2];
+    long* flat = (long*)malloc(sizeof(long) * ncoefs);
+    for (int c = 0; c < ncoefs; c++)
+        flat[c] = (off_c[3 * c] * m[1] + off_c[3 * c + 1]) * m[2] +
+                  off_c[3 * c + 2];
+    const double coef = 1.0 / coefs[0];
+    double* buf = (double*)malloc(sizeof(double) * m[0] * m[1] * m[2]);
+    for (long x = 0; x < nb; x++) {
+        double* b = fun + x * ng;
+        const double* sr = src + x * ng;
+        for (int it = 0; it < nrelax; it++) {
+            orc_halo_fill(b, n, lo, hi, pbc, corners, buf);
+            long p = 0;
+            for (long i0 = 0; i0 < n[0]; i0++)
+                for (long i1 = 0; i1 < n[1]; i1++) {
+                    double* a = buf + ((i0 + r) * m[1] + i1 + r) * m[2] + r;
+                    for (long i2 = 0; i2 < n[2]; i2++, p++) {
+                        double s = 0.0;
+                        for (int c = 1; c < ncoefs; c++)
+                            s += a[i2 + flat[c]] * coefs[c];
+                        if (method == 1) {
+                            s = (sr[p] - s) * coef;
+                            b[p] = s;
+                            a[i2] = s;
+                        } else {
+                            b[p] = (1.0 - w) * b[p] + w * (sr[p] - s) / coefs[0];
+                        }
+                    }
+                }
+        }
+    }
+    free(buf);
     free(flat);
 }
 

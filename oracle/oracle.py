@@ -38,6 +38,8 @@ def lib():
         sig = {
             'orc_halo_fill': (None, [_dp, _lp, _lp, _lp, _ip, i, _dp]),
             'orc_fd': (None, [i, _dp, _lp, _lp, _ip, l, _dp, _dp]),
+            'orc_relax': (None, [i, i, _dp, _lp, _lp, _ip, l, _dp, _dp, i,
+                                 d]),
             'orc_transform': (None, [i, i, _lp, _lp, _lp, _ip, _ip, _lp, l,
                                      _dp, _dp]),
             'orc_spline_build': (None, [d, i, _dp, _dp]),
@@ -104,6 +106,22 @@ def fd(coefs, offsets_c, pbc_c, in_xg):
     lib().orc_fd(len(coefs), _d(coefs), _l(off), _l(n), _i(pbc), nb,
                  _d(in_xg), _d(out))
     return out
+
+
+def relax(method, coefs, offsets_c, pbc_c, fun_xg, src_xg, nrelax, w=1.0):
+    """Operator.relax (c/operators.c:49-91, c/bmgs/relax.c): returns the
+    relaxed copy of fun_xg."""
+    fun = _f64(fun_xg).copy()
+    src = _f64(src_xg)
+    assert src.shape == fun.shape
+    n = _i64(fun.shape[-3:])
+    nb = int(np.prod(fun.shape[:-3], dtype=np.int64))
+    offs = _i64(np.asarray(offsets_c).reshape(-1))
+    pbc = _i32(np.asarray(pbc_c, bool).astype(int))
+    lib().orc_relax(int(method), len(coefs), _d(_f64(coefs)), _l(offs),
+                    _l(n), _i(pbc), nb, _d(fun), _d(src), int(nrelax),
+                    float(w))
+    return fun
 
 
 def transform(k, interpolate, pad_cd, skip_cd, nout_c, pbc_c, in_xg):

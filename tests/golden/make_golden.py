@@ -228,6 +228,82 @@ def main():
                             'out_sha256': digest(d0),
                             'sample': hexbits(d0[1, 5, 6, :8])}
 
+    # Operator.relax (c/operators.c:49-91), both methods, on the Poisson
+    # stencil (scale -1/(4 pi)); then three V-cycles of the FD Poisson
+    # solver (gpaw/poisson.py:278-311) driven on the reference's C objects
+    from math import pi
+    relax = []
+    for N, pbc, nn in (((16, 20, 24), (1, 1, 1), 3), ((20, 16, 12), (1, 0, 1), 3),
+                       ((12, 16, 8), (0, 0, 0), 1)):
+        gd = grid(N, pbc)
+        c, o, _ = guc_laplace_stencil(gd, -0.25 / pi, nn)
+        op = ref_operator(R, gd, c, o)
+        seed = 700 + len(relax)
+        rng = np.random.default_rng(seed)
+        f = rng.standard_normal(tuple(gd.n_c))
+        src = rng.standard_normal(tuple(gd.n_c))
+        fj = f.copy()
+        op.relax(2, fj, src, 3, 2.0 / 3.0)
+        fg = f.copy()
+        op.relax(1, fg, src, 2)
+        relax.append({'N': list(N), 'pbc': list(pbc), 'nn': nn, 'seed': seed,
+                      'jacobi_w': (2.0 / 3.0).hex(), 'jacobi_sweeps': 3,
+                      'gs_sweeps': 2, 'jacobi_sha256': digest(fj),
+                      'gs_sha256': digest(fg)})
+    fx['relax'] = relax
+
+    poisson = []
+    for N, pbc in (((32, 32, 32), (1, 1, 1)), ((16, 24, 32), (0, 0, 0))):
+        gd = grid(N, pbc)
+        scale = -0.25 / pi
+        gds = [gd]
+        ops = [ref_operator(R, gd, *guc_laplace_stencil(gd, scale, 3)[:2])]
+        rst, itp, pre, post, wts = [], [], [2], [1], [2.0 / 3.0]
+        g = gd
+        while len(gds) < 9:
+            try:
+                g2 = g.coarsen()
+            except ValueError:
+                break
+            ops.append(ref_operator(R, g2, *guc_laplace_stencil(g2, scale,
+                                                                1)[:2]))
+            rst.append(ref_transformer(R, g, g2, 1))
+            itp.append(ref_transformer(R, g2, g, 1))
+            pre.append(4)
+            post.append(4)
+            wts.append(1.0)
+            gds.append(g2)
+            g = g2
+        levels = len(gds) - 1
+        pre[levels] = 8
+        post[levels] = 8
+        seed = 800 + len(poisson)
+        rho = np.random.default_rng(seed).standard_normal(tuple(gd.n_c))
+        rho -= rho.mean()
+        phis = [np.zeros(tuple(gd.n_c))] + [None] * levels
+        rhos = [rho.copy()] + [None] * levels
+
+        def iterate2(level=0):
+            if level < levels:
+                ops[level].relax(2, phis[level], rhos[level], pre[level],
+                                 wts[level])
+                res = apply(ops[level], phis[level], gds[level].n_c)
+                res -= rhos[level]
+                rhos[level + 1] = apply(rst[level], res, gds[level + 1].n_c)
+                phis[level + 1] = np.zeros(tuple(gds[level + 1].n_c))
+                iterate2(level + 1)
+                res = apply(itp[level], phis[level + 1], gds[level].n_c)
+                phis[level] -= res
+            ops[level].relax(2, phis[level], rhos[level], post[level],
+                             wts[level])
+
+        for it in range(3):
+            iterate2()
+        poisson.append({'N': list(N), 'pbc': list(pbc), 'seed': seed,
+                        'levels': levels, 'vcycles': 3,
+                        'phi_sha256': digest(phis[0])})
+    fx['poisson'] = poisson
+
     with open(os.path.join(HERE, 'reference_fixtures.json'), 'w') as fh:
         json.dump(fx, fh, indent=1)
     print('wrote', os.path.join(HERE, 'reference_fixtures.json'))

@@ -1,0 +1,126 @@
+"""The multigrid Poisson solver on the device (gpaw/poisson.py): Jacobi
+relaxation and whole V-cycles bit for bit with the reference's compiled
+Operator / Transformer (golden digests), convergence against the oracle's
+restatement, the Mehrstellen variant, and the unsupported paths."""
+import hashlib
+import json
+import os
+from math import pi
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FX = json.load(open(os.path.join(os.path.dirname(__file__), 'golden',
+                                 'reference_fixtures.json')))
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.float64).tobytes()
+                          ).hexdigest()
+
+
+def grid(N, pbc, h=0.2):
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    N = np.array(N)
+    return GridDescriptor(N, N * h / FX['bohr'], pbc)
+
+
+@pytest.mark.parametrize('case', range(3))
+def test_jacobi_relax_matches_reference(cuda, case):
+    import torch
+    from gpawdft_amd.fd_operators import Laplace
+    rx = FX['relax'][case]
+    gd = grid(rx['N'], rx['pbc'])
+    op = Laplace(gd, -0.25 / pi, rx['nn'])
+    rng = np.random.default_rng(rx['seed'])
+    f = rng.standard_normal(tuple(gd.n_c))
+    src = rng.standard_normal(tuple(gd.n_c))
+    df = torch.from_numpy(f).cuda()
+    ds = torch.from_numpy(src).cuda()
+    op.relax(2, df, ds, rx['jacobi_sweeps'], float.fromhex(rx['jacobi_w']))
+    assert digest(df.cpu().numpy()) == rx['jacobi_sha256']
+    # FP32 follows to single precision
+    f32 = torch.from_numpy(f.astype(np.float32)).cuda()
+    op.relax(2, f32, ds.float(), rx['jacobi_sweeps'],
+             float.fromhex(rx['jacobi_w']))
+    ref = df.cpu().numpy()
+    assert np.abs(f32.double().cpu().numpy() - ref).max() < \
+        1e-5 * np.abs(ref).max()
+    with pytest.raises(NotImplementedError):
+        op.relax(1, df, ds, 1)
+
+
+@pytest.mark.parametrize('case', range(2))
+def test_poisson_vcycles_match_reference(cuda, case):
+    import torch
+    from gpawdft_amd.poisson import PoissonSolver
+    px = FX['poisson'][case]
+    gd = grid(px['N'], px['pbc'])
+    ps = PoissonSolver()
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    assert ps.levels == px['levels']
+    rho = np.random.default_rng(px['seed']).standard_normal(tuple(gd.n_c))
+    rho -= rho.mean()
+    phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    ps.phis[0] = phi
+    ps.rhos[0].copy_(torch.from_numpy(rho))
+    for it in range(px['vcycles']):
+        ps.iterate2(ps.step)
+    assert digest(phi.cpu().numpy()) == px['phi_sha256']
+
+
+def test_poisson_solve_matches_oracle_solver(cuda):
+    import torch
+    import poisson as P
+    from gpawdft_amd.poisson import PoissonSolver
+    gd = grid((32, 32, 32), (1, 1, 1))
+    x = np.arange(32) * 2 * np.pi / 32
+    rho = (np.cos(x)[:, None, None] * np.sin(2 * x)[None, :, None] *
+           np.cos(x)[None, None, :] + 0.3)           # charged: background
+    ps = PoissonSolver(eps=1e-18)
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    niter = ps.solve(phi, torch.from_numpy(rho).cuda())
+    ref = P.Poisson(gd, eps=1e-18)
+    phi_ref = np.zeros(tuple(gd.n_c))
+    niter_ref = ref.solve_neutral(phi_ref, rho - rho.mean())
+    assert abs(niter - niter_ref) <= 1
+    got = phi.cpu().numpy()
+    assert np.abs(got - phi_ref).max() < 1e-9 * np.abs(phi_ref).max()
+    assert abs(got.mean()) < 1e-12 * np.abs(got).max()
+
+
+def test_poisson_mehrstellen_and_unsupported(cuda):
+    import torch
+    from gpawdft_amd.fd_operators import mehrstellen_stencils
+    from gpawdft_amd.poisson import PoissonSolver
+    gd = grid((32, 32, 32), (1, 1, 1))
+    rng = np.random.default_rng(3)
+    rho = rng.standard_normal(tuple(gd.n_c))
+    rho -= rho.mean()
+    ps = PoissonSolver(nn='M', eps=1e-20)
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    phi = torch.zeros(tuple(gd.n_c), dtype=torch.float64, device='cuda')
+    ps.solve(phi, torch.from_numpy(rho).cuda())
+    (ca, oa), (cb, ob) = mehrstellen_stencils(gd, 0.25 / pi)
+    p = phi.cpu().numpy()
+    lhs = O.fd(ca, oa, gd.pbc_c, p[None])[0]
+    rhs = O.fd(cb, ob, gd.pbc_c, rho[None])[0]
+    assert np.abs(lhs - rhs).max() < 1e-8 * np.abs(rhs).max()
+    with pytest.raises(NotImplementedError):
+        PoissonSolver(relax='GS')
+    gd0 = grid((16, 16, 16), (0, 0, 0))
+    ps0 = PoissonSolver()
+    ps0.set_grid_descriptor(gd0)
+    ps0.initialize()
+    n = tuple(int(v) for v in gd0.n_c)   # zero boundaries: N - 1 points
+    with pytest.raises(NotImplementedError):
+        ps0.solve(torch.zeros(n, dtype=torch.float64, device='cuda'),
+                  torch.ones(n, dtype=torch.float64, device='cuda'))

@@ -293,3 +293,35 @@ def test_projector_coefs_is_P_times_dH():
     ref[:, 9:14] = P[:, 9:14] @ dH1
     assert np.abs(c - ref).max() < 1e-14 * np.abs(ref).max()
     assert np.abs(c[:, :9] - P[:, :9] @ dH0.T).max() > 1e-3
+
+
+def test_poisson_restatement_converges_to_the_fd_solution():
+    """The oracle's multigrid Poisson solver (oracle/poisson.py) converges
+    to the Fourier-space solution of the same FD equation
+    -1/(4 pi) L phi = rho - <rho> on a periodic grid (zero mean)."""
+    import poisson as P
+    gd = GridDescriptor((16, 16, 16), (6.0, 6.0, 6.0), True)
+    rng = np.random.default_rng(51)
+    x = np.arange(16) * 2 * np.pi / 16
+    rho = (np.cos(x)[:, None, None] * np.sin(2 * x)[None, :, None] *
+           np.cos(3 * x)[None, None, :] + 0.1 * rng.standard_normal((16,) * 3))
+    rho -= rho.mean()
+    ps = P.Poisson(gd, eps=1e-24)
+    phi = np.zeros((16, 16, 16))
+    niter = ps.solve_neutral(phi, rho)
+    assert niter < 200
+    c, o, _ = guc_laplace_stencil(gd, -0.25 / np.pi, 3)
+    k = [2 * np.pi * np.fft.fftfreq(16)] * 3
+    lam = 0.0
+    for ax in range(3):
+        sel = [i for i, oo in enumerate(o) if oo[ax] != 0]
+        sym = sum(c[i] * np.cos(o[i][ax] * k[ax]) for i in sel)
+        shape = [1, 1, 1]
+        shape[ax] = 16
+        lam = lam + sym.reshape(shape)
+    lam = lam + c[0]
+    F = np.fft.fftn(rho)
+    lam.flat[0] = 1.0
+    F.flat[0] = 0.0
+    phi_ref = np.real(np.fft.ifftn(F / lam))
+    assert np.abs(phi - phi_ref).max() < 1e-9 * np.abs(phi_ref).max()

@@ -43,6 +43,8 @@ def parse():
     p.add_argument('--no-ragged', action='store_true',
                    help='skip the ragged-batch graph measurement')
     p.add_argument('--ragged-systems', type=int, default=48)
+    p.add_argument('--no-poisson', action='store_true',
+                   help='skip the multigrid Poisson solver timing')
     p.add_argument('--copy-calibration', action='store_true',
                    help='also run a plain copy of one band block (for '
                         'counter calibration under rocprofv3 --pmc)')
@@ -172,6 +174,10 @@ def main():
     if not args.no_ragged:
         ragged = ragged_batch(args.ragged_systems, args.steps)
 
+    poisson = None
+    if not args.no_poisson:
+        poisson = poisson_solve(2 * N, args.no_cpu)
+
     ttt = None
     if not args.no_ttt:
         ttt = time_to_1e8(args.ttt_max_iter)
@@ -213,6 +219,7 @@ def main():
         'cpu_baseline': cpu,
         'time_to_1e-8': ttt,
         'ragged_batch': ragged,
+        'poisson': poisson,
         'f32_detail': {k: f32[k] for k in ('ms_stencil', 'ms_projections',
                                            'ms_projector_add', 'precond_ms',
                                            'precond_value',
@@ -233,6 +240,51 @@ def main():
         line['hpsi_plus_precond_speedup_vs_cpu_f32'] = (
             f32['hpsi_plus_precond_value'] / cpu['hpsi_plus_precond_value'])
     print(json.dumps(line))
+
+
+def poisson_solve(N, no_cpu):
+    """FP64 multigrid Poisson solve (gpaw/poisson.py, Jacobi smoothing,
+    nn = 3) of a charged periodic density on the N^3 fine grid of the H psi
+    system (twice its points per axis, as GPAW's finegd), tolerance 2e-10;
+    the CPU oracle's V-cycle timed on one cycle for comparison."""
+    import numpy as np
+    import torch
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.poisson import PoissonSolver
+    from gpawdft_amd.synthetic import Bohr
+    n_c = np.array([N, N, N])
+    gd = GridDescriptor(n_c, n_c * 0.09 / Bohr, True)
+    x = np.arange(N) * 2 * np.pi / N
+    rho = (np.cos(x)[:, None, None] * np.sin(2 * x)[None, :, None] *
+           np.cos(3 * x)[None, None, :] + 0.1)
+    ps = PoissonSolver()
+    ps.set_grid_descriptor(gd)
+    ps.initialize()
+    drho = torch.from_numpy(rho).cuda()
+    phi = torch.zeros_like(drho)
+    ps.solve(phi, drho)                      # warm up (tables, scratch)
+    phi.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    niter = ps.solve(phi, drho)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    out = {'workload': 'FP64 %d^3 periodic, nn=3, Jacobi, eps 2e-10, '
+                       '%d levels' % (N, ps.levels + 1),
+           'v_cycles': niter, 'seconds': t, 'ms_per_v_cycle': t / niter * 1e3}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+        import poisson as P
+        ref = P.Poisson(gd)
+        phis = [np.zeros(tuple(n_c))] + [None] * ref.levels
+        rhos = [rho - rho.mean()] + [None] * ref.levels
+        t0 = time.perf_counter()
+        ref.iterate2(phis, rhos)
+        tc = time.perf_counter() - t0
+        out['cpu_ms_per_v_cycle'] = tc * 1e3
+        out['cpu_sample'] = 'one V-cycle of the C oracle (1 thread)'
+        out['speedup_per_v_cycle'] = tc / (t / niter)
+    return out
 
 
 def ragged_batch(nsys, steps):

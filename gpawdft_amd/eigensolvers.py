@@ -7,7 +7,10 @@ Mirrors, for one k-point / spin on one GPU:
   `.iterate` (gpaw/eigensolvers/eigensolver.py:50-172);
 * `RMMDIIS` (gpaw/eigensolvers/rmmdiis.py:10-344): block RMM-DIIS with
   the preconditioned residual step, the lambda line search, DIIS over
-  `niter` vectors and the final trial step.
+  `niter` vectors and the final trial step;
+* `Davidson` (gpaw/eigensolvers/davidson.py:9-262): the subspace of the
+  bands and their preconditioned residuals, 2N x 2N generalized
+  eigenproblem, rotation into the lowest N.
 
 The error measure is the reference's: sum_n w_n <R_n|R_n> (dv included).
 Every grid-sized operation runs on the device (stencil, projectors,
@@ -126,6 +129,8 @@ class Overlap:
 
 
 class Eigensolver:
+    orthonormalization_required = True
+
     def __init__(self, keep_htpsit=True, blocksize=1):
         self.keep_htpsit = keep_htpsit
         self.blocksize = blocksize
@@ -146,7 +151,10 @@ class Eigensolver:
             wfs.overlap.orthonormalize(wfs, kpt)
             wfs.orthonormalized = True
         error, psit_nG = self.iterate_one_k_point(ham, wfs, kpt)
-        wfs.overlap.orthonormalize(wfs, kpt, psit_nG)
+        if self.orthonormalization_required:
+            wfs.overlap.orthonormalize(wfs, kpt, psit_nG)
+        else:
+            kpt.psit_nG = psit_nG
         self.error = error
         return error
 
@@ -371,3 +379,101 @@ class RMMDIIS(Eigensolver):
         for i in range(m):
             axpy(alpha[:, i], pd[:, i].contiguous(), psit_x)
             axpy(alpha[:, i], rcols[i], R_x)
+
+
+class Davidson(Eigensolver):
+    """Simple Davidson (gpaw/eigensolvers/davidson.py:9-262): after the
+    subspace diagonalisation, the preconditioned residuals psi2 (normalized)
+    extend the basis; H and S in the 2N basis (lower blocks: <psi2|H|psi>,
+    <psi2|H|psi2>, diag(eps); S likewise with 1) go to the FP64
+    generalized eigensolver, and the lowest N eigenvectors rotate psi and
+    psi2 into the new bands (b-orthonormal: no separate
+    orthonormalisation)."""
+
+    orthonormalization_required = False
+
+    def __init__(self, niter=1, smin=None, normalize=True, blocksize=16):
+        # blocksize only batches the preconditioner calls (per-band
+        # results do not depend on it)
+        Eigensolver.__init__(self, blocksize=blocksize)
+        if smin is not None:
+            raise NotImplementedError('smin is not supported (as in the '
+                                      'reference)')
+        self.niter = niter
+        self.smin = smin
+        self.normalize = normalize
+
+    def todict(self):
+        return {'name': 'dav', 'niter': self.niter}
+
+    def iterate_one_k_point(self, ham, wfs, kpt):
+        from gpawdft_amd.hs_operators import BandLayouts
+        dv = wfs.gd.dv
+        nb = wfs.nbands
+        pt = wfs.pt
+        op = wfs.matrixoperator
+        psit_nG, R_nG = self.subspace_diagonalize(ham, wfs, kpt)
+        P = kpt.P_xM
+        self.calculate_residuals(kpt, wfs, ham, psit_nG, P, kpt.eps_n, R_nG)
+        dH = {a: unpack(ham.dH_asp[a][kpt.s]) for a in pt.atom_indices}
+        pc = self.preconditioner
+        ksl2 = BandLayouts(wfs.gd, 2 * nb)
+        dev = psit_nG.device
+        error = 0.0
+        for nit in range(self.niter):
+            if kpt.f_n is None:
+                w_n = np.full(nb, kpt.weight)
+            else:
+                w_n = np.asarray(kpt.f_n)
+            error = float(np.sum(w_n * dots(R_nG, R_nG, dv)))
+            psit2_nG = torch.empty_like(psit_nG)
+            for n1 in range(0, nb, pc.block):
+                n2 = min(n1 + pc.block, nb)
+                pc(R_nG[n1:n2], kpt, None, out=psit2_nG[n1:n2])
+            if self.normalize:
+                norm_n = dots(psit2_nG, psit2_nG, dv)
+                scaled = torch.zeros_like(psit2_nG)
+                axpy(norm_n**-0.5, psit2_nG, scaled)
+                psit2_nG = scaled
+            P2 = wfs.zeros_P(nb)
+            wfs.project(psit2_nG, P2)
+
+            def Hop(x):
+                # H psi2 into the residual storage, reused for <psi2|H|psi2>
+                wfs.apply_pseudo_hamiltonian(ham, x, R_nG)
+                return R_nG
+
+            H_2 = torch.zeros((2 * nb, 2 * nb), dtype=torch.float64,
+                              device=dev)
+            S_2 = torch.zeros_like(H_2)
+            idx = torch.arange(nb, device=dev)
+            H_2[idx, idx] = torch.as_tensor(np.asarray(kpt.eps_n),
+                                            dtype=torch.float64, device=dev)
+            S_2[idx, idx] = 1.0
+            # rows: psi2 bands (the reference's lower blocks)
+            H_2[nb:, :nb] = op.calculate_matrix_elements(
+                psit_nG, P, Hop, dH, psit2_nG, P2, pt=pt)
+            H_2[nb:, nb:] = op.calculate_matrix_elements(
+                psit2_nG, P2, lambda x: R_nG, dH, pt=pt)
+            dO = wfs.dO_aii
+            S_2[nb:, :nb] = op.calculate_matrix_elements(
+                psit_nG, P, lambda x: x, dO, psit2_nG, P2, pt=pt)
+            S_2[nb:, nb:] = op.calculate_matrix_elements(
+                psit2_nG, P2, lambda x: x, dO, pt=pt)
+            eps_2n = np.empty(2 * nb)
+            ksl2.general_diagonalize(H_2, eps_2n, S_2)
+            kpt.eps_n[:] = eps_2n[:nb]
+            C1 = H_2[:nb, :nb].contiguous()
+            C2 = H_2[:nb, nb:].contiguous()
+            new = op.matrix_multiply(C1, psit_nG, P)
+            tmp = op.matrix_multiply(C2, psit2_nG, P2)
+            axpy(np.ones(nb), tmp, new)
+            P += P2
+            psit_nG = new
+            kpt.P_xM = P
+            if nit < self.niter - 1:
+                R_nG = torch.empty_like(psit_nG)
+                wfs.apply_pseudo_hamiltonian(ham, psit_nG, R_nG)
+                self.calculate_residuals(kpt, wfs, ham, psit_nG, P,
+                                         kpt.eps_n, R_nG)
+        return error, psit_nG

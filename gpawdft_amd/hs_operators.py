@@ -105,6 +105,31 @@ def diagonalize(a, w):
     return sweeps.value
 
 
+def _matmul(C, A):
+    """C @ A for FP64 device (N, N) matrices on the band-rotation kernel."""
+    N = A.shape[0]
+    out = torch.empty_like(A)
+    rotate(C, A.contiguous().reshape(N, 1, 1, -1), out.reshape(N, 1, 1, -1))
+    return out
+
+
+def general_diagonalize(a, w, b):
+    """Generalized symmetric eigenproblem a x = lambda b x
+    (gpaw/utilities/lapack.py:62-108, dsygvd): lower triangles of a and b
+    (FP64 device (n, n)) are used; eigenvalues ascending into w, the
+    b-orthonormal eigenvectors into the rows of a.  b is left holding
+    L^-1 (b = L L^T).  Steps: inverse Cholesky of b, a' = L^-1 a L^-T,
+    Jacobi eigensolve of a', eigenvectors V L^-1."""
+    n = a.shape[0]
+    assert a.shape == (n, n) and b.shape == (n, n) and w.shape == (n,)
+    inverse_cholesky(b)
+    full = torch.tril(a) + torch.tril(a, -1).t()
+    t = _matmul(b, full)
+    ap = _matmul(b, t.t().contiguous())
+    diagonalize(ap, w)
+    a.copy_(_matmul(ap, b))
+
+
 class BandLayouts:
     """Serial (one GPU) layouts: the dense FP64 steps on the device."""
 
@@ -124,6 +149,15 @@ class BandLayouts:
 
     def inverse_cholesky(self, S_NN):
         inverse_cholesky(S_NN)
+
+    def general_diagonalize(self, H_NN, eps_n, S_NN):
+        w = torch.empty(H_NN.shape[0], dtype=torch.float64,
+                        device=H_NN.device)
+        general_diagonalize(H_NN, w, S_NN)
+        if isinstance(eps_n, np.ndarray):
+            eps_n[:] = w.cpu().numpy()
+        else:
+            eps_n.copy_(w)
 
     def get_description(self):
         return 'Serial GPU (Jacobi eigensolver, inverse Cholesky)'
@@ -171,7 +205,10 @@ class MatrixOperator:
 
     def calculate_matrix_elements(self, psit1_nG, P1_ani, A, dA,
                                   psit2_nG=None, P2_ani=None, pt=None):
-        """A_NN = dv <psi1|A|psi2> + sum_a P1 dA_a P2^T (FP64 device).
+        """A_NN[n2][n1] = dv <A psit2_n2 | psit1_n1> + sum_a (P2 dA P1^T)
+        (FP64 device), rows indexed by psit2's bands as in the reference
+        (gpaw/hs_operators.py:304-311: integrate(psit1, A psit2) into the
+        transposed result, gemm(1.0, P1, dA(P2), 1.0, A_NN, 'c')).
 
         P1_ani / P2_ani: a (nbands, Mmax) device tensor of projections, or
         None for no PAW term; `pt` (the LFC) locates the atoms' columns."""
@@ -188,7 +225,7 @@ class MatrixOperator:
         if self.A_nn is None or self.A_nn.shape != (N, N):
             self.A_nn = torch.empty((N, N), dtype=torch.float64, device=dev)
         same = Apsit is psit1_nG
-        grid_inner_products(self.gd, psit1_nG, psit1_nG if same else Apsit,
+        grid_inner_products(self.gd, psit1_nG if same else Apsit, psit1_nG,
                             hermitian=hermitian, _transposed_result=self.A_nn)
         if P1_ani is not None and pt is not None and pt.Mmax:
             natoms, M_a, ni_a, off, D = _dA_tables(pt, dA, dev)

@@ -234,3 +234,36 @@ def test_mixed_precision_subspace_step_112x512(cuda):
           % (s_err, e_err, eps64.min(), eps64.max()))
     assert s_err < 1e-6
     assert e_err < 2e-6
+
+
+def test_matrix_elements_between_two_band_sets(cuda):
+    """With distinct band sets the matrix keeps the reference's orientation
+    (rows: psit2's bands): A[n2][n1] = dv <A psit2_n2|psit1_n1> +
+    sum_a P2 dA P1^T, grid and PAW parts alike."""
+    import torch
+    from gpawdft_amd.hs_operators import BandLayouts, MatrixOperator
+    from gpawdft_amd.synthetic import System
+    sysm = System(24, 0.18, 3, seed=8)
+    kin, pt = sysm.operators()
+    N = 10
+    p1 = dev(smooth_orbitals(N, sysm.gd.n_c, 21))
+    p2 = dev(smooth_orbitals(N, sysm.gd.n_c, 22))
+    P1 = torch.zeros((N, pt.Mmax), dtype=torch.float64, device='cuda')
+    P2 = torch.zeros_like(P1)
+    pt.integrate_xM(p1, P1)
+    pt.integrate_xM(p2, P2)
+    rng = np.random.default_rng(23)
+    dA = {a: rng.standard_normal((9, 9)) for a in range(3)}
+    Ap2 = torch.empty_like(p2)
+    kin.apply(p2, Ap2)
+    op = MatrixOperator(BandLayouts(sysm.gd, N))
+    A = op.calculate_matrix_elements(p1, P1, lambda x: Ap2, dA, p2, P2,
+                                     pt=pt).cpu().numpy()
+    x1 = p1.cpu().numpy().reshape(N, -1)
+    y2 = Ap2.cpu().numpy().reshape(N, -1)
+    ref = sysm.gd.dv * y2 @ x1.T
+    h1, h2 = P1.cpu().numpy(), P2.cpu().numpy()
+    for a in range(3):
+        M1, M2 = pt.M_range(a)
+        ref += h2[:, M1:M2] @ dA[a] @ h1[:, M1:M2].T
+    assert np.abs(A - ref).max() < 1e-12 * np.abs(ref).max()

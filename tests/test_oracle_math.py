@@ -325,3 +325,32 @@ def test_poisson_restatement_converges_to_the_fd_solution():
     F.flat[0] = 0.0
     phi_ref = np.real(np.fft.ifftn(F / lam))
     assert np.abs(phi - phi_ref).max() < 1e-9 * np.abs(phi_ref).max()
+
+
+def test_davidson_restatement_converges_to_dense_spectrum():
+    """The numpy Davidson (oracle/davidson.py) from a start inside its basin
+    converges the lowest bands to the dense eigenvalues of the oracle's FD
+    PAW Hamiltonian, and keeps the bands orthonormal."""
+    import davidson as DV
+    import rmmdiis as RM
+    from helpers import dense_hamiltonian, near_eigenvector_start, \
+        oracle_preconditioner
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import System
+    sysm = System(12, 0.3, 2, seed=11, unit_projectors=True)
+    pt = LFC(sysm.gd, [sysm.splines] * sysm.natoms)
+    pt.set_positions(sysm.spos_ac)
+    H = dense_hamiltonian(sysm, pt)
+    w, psi = near_eigenvector_start(H, 8, sysm.gd)
+    pc = oracle_preconditioner(sysm.gd)
+    psi = RM.orthonormalize(psi, sysm.gd.dv)
+    errs = []
+    for it in range(30):
+        err, eps, psi = DV.davidson_iteration(H, psi, sysm.gd.dv, pc)
+        errs.append(err)
+    # niter = 1 Davidson converges more slowly than RMM-DIIS; the top
+    # bands lag
+    assert errs[-1] < 1e-5 * errs[0]
+    assert np.abs(eps[:5] - w[:5]).max() < 1e-9
+    S = sysm.gd.dv * psi @ psi.T
+    assert np.abs(S - np.eye(8)).max() < 1e-10

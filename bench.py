@@ -327,19 +327,27 @@ def time_to_1e8(max_iter):
     """Wall-clock seconds for FP64 RMM-DIIS to bring the occupied-band
     residual error below 1e-8 with the multigrid preconditioner and with the
     learned one (multigrid fallback), on a held-out 32^3 system of the same
-    generators (h = 0.18 A, 3 atoms, 16 bands, 12 occupied), same start."""
+    generators (h = 0.18 A, 3 wells, unit-norm projectors, 16 bands, 12
+    occupied), both from the same Gaussian-orbital start.  Both runs must
+    end on the same eigenvalues for the times to compare like with like:
+    eps_max_diff is the largest difference of their occupied eigenvalues."""
+    import numpy as np
     from gpawdft_amd.learned import UNet
     from gpawdft_amd.precond_training import time_to_tolerance
     from gpawdft_amd.synthetic import System
     net = UNet.load(os.path.join(ROOT, 'gpawdft_amd', 'data',
                                  'unet_precond.npz'))
-    sysm = System(32, 0.18, 3, seed=4242)
-    out = {'workload': '32^3 periodic grid, h=0.18 A, 3 atoms, 16 FP64 '
-                       'bands (12 occupied), blocksize 8, tolerance 1e-8, '
-                       'at most %d iterations' % max_iter}
+    sysm = System(32, 0.18, 3, seed=4242, unit_projectors=True)
+    out = {'workload': '32^3 periodic grid, h=0.18 A, 3 wells, unit-norm '
+                       'projectors, 16 FP64 bands (12 occupied) from '
+                       'Gaussian orbitals on the wells, blocksize 8, '
+                       'tolerance 1e-8, at most %d iterations' % max_iter}
     for name, n in (('multigrid', None), ('learned', net)):
         out[name] = time_to_tolerance(sysm, n, nbands=16, nocc=12,
                                       max_iter=max_iter)
+    e_m = np.array(out['multigrid']['eps_n'][:12])
+    e_l = np.array(out['learned']['eps_n'][:12])
+    out['eps_max_diff'] = float(np.abs(e_m - e_l).max())
     return out
 
 

@@ -187,12 +187,15 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
         for (int s = 0; s < W - 1; s++) win[k][s] = load_plane(z0 - R + s, own_off[k]);
         win[k][W - 1] = T(0);
     }
-    T ringv[2][RPT];
+    // The halo ring of plane z0 + i sits in the window's slot for i, so it
+    // is fetched in the same step as the neighbouring tiles fetch those
+    // rows as their own (they are then still in L2)
+    T ringv[W][RPT];
 #pragma unroll
-    for (int q = 0; q < 2; q++)
+    for (int s = 0; s < R + D; s++)
 #pragma unroll
         for (int k = 0; k < RPT; k++)
-            ringv[q][k] = ring_i[k] >= 0 ? load_plane(z0 + q, ring_off[k]) : T(0);
+            ringv[(s + R) % W][k] = ring_i[k] >= 0 ? load_plane(z0 + s, ring_off[k]) : T(0);
 
     for (int ib = 0; ib < nz; ib += W) {
 #pragma unroll
@@ -207,18 +210,19 @@ __global__ __launch_bounds__(kThreads) void fd_axis_kernel(
                 lds[buf][R + ty + 8 * k][R + tx] = win[k][(ph + R) % W];
 #pragma unroll
             for (int k = 0; k < RPT; k++)
-                if (ring_i[k] >= 0) lds[buf][ring_i[k]][ring_j[k]] = ringv[ph & 1][k];
+                if (ring_i[k] >= 0) lds[buf][ring_i[k]][ring_j[k]] = ringv[(ph + R) % W][k];
             lds_barrier();
-            // prefetch: window plane z + R + D, ring plane z + 2
+            // prefetch plane z + R + D: own rows, and its ring while the
+            // plane is inside the chunk
             if (i + D < nz) {
 #pragma unroll
                 for (int k = 0; k < kRows; k++)
                     win[k][(ph + W - 1) % W] = load_plane(z + R + D, own_off[k]);
             }
-            if (i + 2 < nz) {
+            if (i + R + D < nz) {
 #pragma unroll
                 for (int k = 0; k < RPT; k++)
-                    if (ring_i[k] >= 0) ringv[ph & 1][k] = load_plane(z + 2, ring_off[k]);
+                    if (ring_i[k] >= 0) ringv[(ph + W - 1) % W][k] = load_plane(z + R + D, ring_off[k]);
             }
             // compute
 #pragma unroll

@@ -179,26 +179,36 @@ def residual_reduction(net_or_none, mg, kin, X, lam, R, eps_t):
 
 
 def time_to_tolerance(sysm, net, nbands=16, nocc=12, tol=1e-8,
-                      max_iter=150, dtype=torch.float64, seed=5, blocksize=8):
+                      max_iter=150, dtype=torch.float64, seed=5, blocksize=8,
+                      start='atomic', bf16=True):
     """Wall-clock seconds and iterations RMM-DIIS needs to bring the error
     sum_n f_n <R_n|R_n> (f_n = 1 for the lowest nocc bands, 0 above: the
     reference's convergence over occupied bands) below tol, with the
     multigrid preconditioner (net None) or the learned one with multigrid
-    fallback.  Starts from smoothed random bands."""
+    fallback.  start='atomic' begins from Gaussian orbitals on the wells
+    (RMM-DIIS converges each band to a nearby eigenvector, so a start with
+    little weight on the lowest states can converge to excited ones),
+    'noise' from smoothed random bands (seed).  Returns the final
+    eigenvalues too."""
     from gpawdft_amd.learned import LearnedPreconditioner
+    from gpawdft_amd.synthetic import atomic_start
     kin, pt = sysm.operators()
     ham = Hamiltonian(sysm.gd, [sysm.vt_G], sysm.dH_asp)
     factory = None
     if net is not None:
         def factory(w, b):
             return LearnedPreconditioner(
-                Preconditioner(sysm.gd, kin, block=b), net, sysm.vt_G)
+                Preconditioner(sysm.gd, kin, block=b), net, sysm.vt_G,
+                bf16=bf16)
     wfs = WaveFunctions(sysm.gd, kin, pt, nbands, dtype,
                         preconditioner_factory=factory)
     f_n = np.zeros(nbands)
     f_n[:nocc] = 1.0
-    kpt = KPoint(smooth_bands_device(nbands, sysm.gd.n_c, seed, dtype=dtype),
-                 f_n=f_n)
+    if start == 'atomic':
+        psi = atomic_start(sysm.gd, sysm.wells, nbands, dtype=dtype)
+    else:
+        psi = smooth_bands_device(nbands, sysm.gd.n_c, seed, dtype=dtype)
+    kpt = KPoint(psi, f_n=f_n)
     es = RMMDIIS(blocksize=blocksize)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -211,7 +221,8 @@ def time_to_tolerance(sysm, net, nbands=16, nocc=12, tol=1e-8,
     return {'seconds': time.perf_counter() - t0, 'iterations': len(errors),
             'converged': bool(errors[-1] < tol), 'error': errors[-1],
             'learned_kept': es.learned_kept,
-            'learned_offered': es.learned_offered}
+            'learned_offered': es.learned_offered,
+            'eps_n': [float(e) for e in np.asarray(kpt.eps_n)]}
 
 
 def main():

@@ -27,9 +27,10 @@ def make_grid(N, h_ang, pbc=True):
     return GridDescriptor(N_c, N_c * h_ang / Bohr, pbc)
 
 
-def gaussian_wells(gd, natoms, seed):
+def gaussian_wells(gd, natoms, seed, return_params=False):
     """Local potential on gd (FP64 host array) and the well centres
-    (scaled positions)."""
+    (scaled positions); with return_params=True the second item is
+    (centres, depths, widths in Bohr)."""
     rng = np.random.default_rng(seed)
     spos_ac = rng.uniform(0.0, 1.0, (natoms, 3))
     depth_a = rng.uniform(1.0, 5.0, natoms)
@@ -48,6 +49,8 @@ def gaussian_wells(gd, natoms, seed):
             shape[c] = -1
             d2 = d2 + (x**2).reshape(shape)
         v -= depth_a[a] * np.exp(-d2 / (2 * width_a[a]**2))
+    if return_params:
+        return v, (spos_ac, depth_a, width_a)
     return v, spos_ac
 
 
@@ -118,13 +121,53 @@ def smooth_bands_device(nb, n_c, seed, sigma=1.0, dtype=None, device='cuda'):
     return psi.to(dtype).contiguous()
 
 
+def atomic_start(gd, wells, nbands, dtype=None, device='cuda'):
+    """Starting bands built from Gaussian orbitals on the potential wells
+    (the role of the reference's LCAO initial guess): band k sits on well
+    k mod natoms (deepest first) with shell k // natoms of s(w), p_x, p_y,
+    p_z, s(2w), d_xy, d_yz, d_zx, s(3w), ... where w is the well's width.
+    wells: (spos_ac, depth_a, width_a) as returned by gaussian_wells with
+    return_params=True."""
+    import torch
+    spos_ac, depth_a, width_a = wells
+    order = np.argsort(-np.asarray(depth_a))
+    L_c = np.diag(gd.cell_cv)
+    axes = [np.arange(gd.beg_c[c], gd.end_c[c]) * (L_c[c] / gd.N_c[c])
+            for c in range(3)]
+    shells = [('s', 1), ('x', 1), ('y', 1), ('z', 1), ('s', 2), ('xy', 1),
+              ('yz', 1), ('zx', 1)]
+    psi = np.empty((nbands,) + tuple(gd.n_c))
+    for k in range(nbands):
+        a = order[k % len(order)]
+        j = k // len(order)
+        sh, wf = shells[j] if j < len(shells) else ('s', j - len(shells) + 3)
+        R_c = spos_ac[a] * L_c
+        x = []
+        for c in range(3):
+            d = axes[c] - R_c[c]
+            d -= L_c[c] * np.round(d / L_c[c])
+            shape = [1, 1, 1]
+            shape[c] = -1
+            x.append(d.reshape(shape))
+        w = wf * width_a[a]
+        g = np.exp(-(x[0]**2 + x[1]**2 + x[2]**2) / (2 * w**2))
+        for c, name in enumerate('xyz'):
+            if name in sh:
+                g = g * x[c]
+        psi[k] = g / np.sqrt(np.sum(g**2) * gd.dv)
+    dtype = dtype or torch.float64
+    return torch.as_tensor(psi, dtype=dtype, device=device).contiguous()
+
+
 class System:
     """A periodic grid with potential, projectors and dH (one spin)."""
 
     def __init__(self, N, h_ang, natoms, seed=0, nn=3, lmax=2,
                  unit_projectors=False):
         self.gd = make_grid(N, h_ang)
-        self.vt_G, self.well_spos = gaussian_wells(self.gd, natoms, seed)
+        self.vt_G, self.wells = gaussian_wells(self.gd, natoms, seed,
+                                               return_params=True)
+        self.well_spos = self.wells[0]
         rng = np.random.default_rng(seed + 1)
         self.spos_ac = rng.uniform(0.0, 1.0, (natoms, 3))
         self.splines = projector_splines(lmax, normalize=unit_projectors)

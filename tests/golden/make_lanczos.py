@@ -1,0 +1,80 @@
+"""Lowest eigenvalues of synthetic Hamiltonians by Lanczos on the CPU.
+
+H = T + v + sum_a p_a dH_a p_a^T is applied by the FP64 oracle (stencil,
+local potential, projector integrate / dH / add, each pinned to the
+reference's compiled C by the golden digests); scipy's ARPACK (eigsh,
+which='SA', tol 1e-10) finds the lowest eigenvalues.  No device code is
+involved, so the eigensolvers on the device can be checked against
+eigenvalues computed by an independent method.
+
+Systems (synthetic.System, 32^3, h = 0.18 A, 3 wells, seed 4242):
+* unit-norm projectors: the nonlocal term is bounded by dH, the lowest
+  states are bound in the wells;
+* the default, unnormalized projectors: p dH p^T has eigenvalues far below
+  the wells', so the spectrum starts with deep nonlocal states.
+
+    python tests/golden/make_lanczos.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import scipy.sparse.linalg as sla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+
+import oracle as O  # noqa: E402
+from gpawdft_amd.fd_operators import guc_laplace_stencil  # noqa: E402
+from gpawdft_amd.hamiltonian import unpack  # noqa: E402
+from gpawdft_amd.lfc import LFC  # noqa: E402
+from gpawdft_amd.synthetic import System  # noqa: E402
+
+
+def lowest(sysm, k):
+    gd = sysm.gd
+    coefs, offs, _ = guc_laplace_stencil(gd, -0.5, sysm.nn)
+    pt = LFC(gd, [sysm.splines] * sysm.natoms)
+    pt.set_positions(sysm.spos_ac)
+    atoms = [(pt.M_a[a], unpack(sysm.dH_asp[a][0])) for a in pt.atom_indices]
+    shape = tuple(gd.n_c)
+    G = int(np.prod(shape))
+
+    def H(psi):
+        t = O.fd(coefs, offs, gd.pbc_c, psi)
+        t = O.add_local_potential(psi, sysm.vt_G, t)
+        P = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, psi,
+                            pt.Mmax)
+        c = O.projector_coefs(P, atoms)
+        return O.lfc_add(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, c, t)
+
+    op = sla.LinearOperator(
+        (G, G), dtype=float,
+        matvec=lambda v: H(np.ascontiguousarray(v.reshape((1,) + shape)))
+        .ravel(),
+        matmat=lambda V: H(np.ascontiguousarray(V.T.reshape((-1,) + shape)))
+        .reshape(V.shape[1], G).T)
+    w = sla.eigsh(op, k=k, which='SA', tol=1e-10, ncv=64,
+                  return_eigenvectors=False)
+    return sorted(float(x) for x in w)
+
+
+def main():
+    out = {'method': 'scipy.sparse.linalg.eigsh(which="SA", tol=1e-10, '
+                     'ncv=64) on the FP64 oracle H',
+           'systems': []}
+    for unit in (True, False):
+        sysm = System(32, 0.18, 3, seed=4242, unit_projectors=unit)
+        out['systems'].append({'N': 32, 'h_ang': 0.18, 'natoms': 3,
+                               'seed': 4242, 'unit_projectors': unit,
+                               'lowest': lowest(sysm, 16)})
+        print(out['systems'][-1], flush=True)
+    with open(os.path.join(HERE, 'lanczos_eigenvalues.json'), 'w') as fh:
+        json.dump(out, fh, indent=1)
+
+
+if __name__ == '__main__':
+    main()

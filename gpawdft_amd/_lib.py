@@ -16,6 +16,7 @@ GPD_ERR_UNSUPPORTED = -3
 GPD_ERR_NOMEM = -4
 GPD_F64 = 0
 GPD_F32 = 1
+GPD_C128 = 2
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         'libgpawdft_amd.so')

@@ -62,6 +62,75 @@ __global__ void fd_generic_kernel(const T* __restrict__ in, T* __restrict__ out,
 }
 
 // ------------------------------------------------------------------ //
+// complex (Bloch) kernel: one thread per point, any stencil.  A neighbour
+// outside the cell along a periodic axis is the wrapped value times that
+// axis's phase ([axis][0] for the image above, [axis][1] below), the
+// phases applied in axis order with the C99 complex product
+// (re = pr*vr - pi*vi, im = pr*vi + pi*vr, two roundings each); each part
+// is then summed from 0 in coefficient order with two roundings per term,
+// as the reference's phased halo and bmgs_fd for double complex do.
+
+struct BlochPhases {
+    double re[3][2];
+    double im[3][2];
+};
+
+__global__ void fd_complex_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                  int n0, int n1, int n2, int pbc0, int pbc1, int pbc2,
+                                  int ncoefs, const int* __restrict__ off,
+                                  const double* __restrict__ coefs, BlochPhases ph)
+{
+    const long nG = (long)n0 * n1 * n2;
+    const long band = blockIdx.y;
+    const double2* a = in + band * nG;
+    double2* b = out + band * nG;
+    const int n[3] = {n0, n1, n2};
+    const int pbc[3] = {pbc0, pbc1, pbc2};
+    for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < nG;
+         p += (long)gridDim.x * blockDim.x) {
+        const int i[3] = {(int)(p / ((long)n1 * n2)), (int)((p / n2) % n1), (int)(p % n2)};
+        double sr = 0.0, si = 0.0;
+        for (int c = 0; c < ncoefs; c++) {
+            int j[3], side[3];
+            bool inside = true;
+#pragma unroll
+            for (int ax = 0; ax < 3; ax++) {
+                int q = i[ax] + off[3 * c + ax];
+                side[ax] = -1;
+                if (q < 0) {
+                    side[ax] = 1;
+                    q += n[ax];
+                } else if (q >= n[ax]) {
+                    side[ax] = 0;
+                    q -= n[ax];
+                }
+                if (side[ax] >= 0 && !pbc[ax]) inside = false;
+                j[ax] = q;
+            }
+            double vr = 0.0, vi = 0.0;
+            if (inside) {
+                const double2 v = a[((long)j[0] * n1 + j[1]) * n2 + j[2]];
+                vr = v.x;
+                vi = v.y;
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    if (side[ax] >= 0) {
+                        const double pr = ph.re[ax][side[ax]], pim = ph.im[ax][side[ax]];
+                        const double re1 = pr * vr, re2 = pim * vi;
+                        const double im1 = pr * vi, im2 = pim * vr;
+                        vr = re1 - re2;
+                        vi = im1 + im2;
+                    }
+                }
+            }
+            sr = madd<double>(sr, vr, coefs[c]);
+            si = madd<double>(si, vi, coefs[c]);
+        }
+        b[p] = make_double2(sr, si);
+    }
+}
+
+// ------------------------------------------------------------------ //
 // axis kernel
 
 constexpr int kTX = 32;        // tile width along axis 2 (contiguous)
@@ -643,11 +712,37 @@ extern "C" int gpd_operator_apply(gpd_operator* op, int dtype, long nin, const v
                                   void* d_out, const void* phases, void* stream)
 {
     GPD_REQUIRE(op, GPD_ERR_ARG, "null operator");
-    GPD_REQUIRE(!phases, GPD_ERR_UNSUPPORTED, "Bloch phases are not supported");
+    GPD_REQUIRE(!phases || dtype == GPD_C128, GPD_ERR_ARG,
+                "phases apply to complex arrays only");
     GPD_REQUIRE(nin >= 0, GPD_ERR_ARG, "negative batch");
     if (nin == 0) return GPD_OK;
     GPD_REQUIRE(d_in && d_out && d_in != d_out, GPD_ERR_ARG, "in and out must be distinct device arrays");
     hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_C128) {
+        // a neighbour wraps at most once
+        for (int c = 0; c < op->ncoefs; c++)
+            for (int ax = 0; ax < 3; ax++)
+                GPD_REQUIRE(std::abs(op->off[3 * c + ax]) <= op->n[ax], GPD_ERR_ARG,
+                            "stencil reaches beyond one periodic image");
+        BlochPhases ph;
+        for (int ax = 0; ax < 3; ax++)
+            for (int d = 0; d < 2; d++) {
+                const double* q = (const double*)phases + 4 * ax + 2 * d;
+                ph.re[ax][d] = phases ? q[0] : 1.0;
+                ph.im[ax][d] = phases ? q[1] : 0.0;
+            }
+        const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+        const long blocks = std::min<long>((nG + 255) / 256, 4096);
+        for (long b0 = 0; b0 < nin; b0 += 65535) {
+            const long nbb = std::min<long>(65535, nin - b0);
+            hipLaunchKernelGGL(fd_complex_kernel, dim3((unsigned)blocks, (unsigned)nbb),
+                               dim3(256), 0, st, (const double2*)d_in + b0 * nG,
+                               (double2*)d_out + b0 * nG, op->n[0], op->n[1], op->n[2],
+                               op->pbc[0], op->pbc[1], op->pbc[2], op->ncoefs, op->d_off,
+                               op->d_coefs64, ph);
+        }
+        return last_error_to_status(hipGetLastError());
+    }
     if (dtype == GPD_F64) {
         FdEpilogue<double> ep = {};
         return fd_apply_typed<double>(op, EPI_PLAIN, nin, (const double*)d_in, (double*)d_out, ep, st);

@@ -17,7 +17,7 @@ try:
 except ImportError:  # pragma: no cover - torch is part of the image
     torch = None
 
-from gpawdft_amd._lib import GPD_F32, GPD_F64
+from gpawdft_amd._lib import GPD_C128, GPD_F32, GPD_F64
 
 DEFAULT_DEVICE = 'cuda'
 
@@ -34,14 +34,16 @@ def dtype_code(dt):
             return GPD_F64
         if dt == torch.float32:
             return GPD_F32
+        if dt == torch.complex128:
+            return GPD_C128
         raise TypeError('unsupported dtype %s' % dt)
     dt = np.dtype(dt)
     if dt == np.float64:
         return GPD_F64
     if dt == np.float32:
         return GPD_F32
-    if dt.kind == 'c':
-        raise NotImplementedError('complex (Bloch) arrays are not supported')
+    if dt == np.complex128:
+        return GPD_C128
     raise TypeError('unsupported dtype %s' % dt)
 
 
@@ -49,7 +51,8 @@ def torch_dtype(dt):
     _require_torch()
     if isinstance(dt, torch.dtype):
         return dt
-    return {GPD_F64: torch.float64, GPD_F32: torch.float32}[dtype_code(dt)]
+    return {GPD_F64: torch.float64, GPD_F32: torch.float32,
+            GPD_C128: torch.complex128}[dtype_code(dt)]
 
 
 def stream_ptr():

@@ -12,7 +12,7 @@ from math import factorial as fact
 import numpy as np
 
 from gpawdft_amd import _lib
-from gpawdft_amd._lib import check, lib
+from gpawdft_amd._lib import GPD_C128, check, lib
 from gpawdft_amd.device import DeviceView, stream_ptr
 
 # Chelikowsky et al., Phys. Rev. B 50, 11355 (1994): central-difference
@@ -83,9 +83,10 @@ class FDOperator:
         return '<' + self.description + '>'
 
     def apply(self, in_xg, out_xg, phase_cd=None):
-        """out_xg = A in_xg for a batch of grids (last three axes)."""
-        if phase_cd is not None and np.iscomplexobj(np.asarray(phase_cd)):
-            raise NotImplementedError('Bloch phases are not supported')
+        """out_xg = A in_xg for a batch of grids (last three axes).  For
+        complex128 grids phase_cd ((3, 2) complex, gpaw/kpoint.py: [c][0]
+        for the periodic image above, [c][1] below) gives the Bloch
+        boundary condition; None means Gamma (all ones)."""
         if tuple(in_xg.shape) != tuple(out_xg.shape):
             raise ValueError('in/out shapes differ')
         if tuple(in_xg.shape[-3:]) != self.shape:
@@ -96,9 +97,14 @@ class FDOperator:
         if vin.code != vout.code:
             raise TypeError('in/out dtypes differ')
         nin = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
-        check(lib().gpd_operator_apply(self._handle, vin.code, nin, vin.ptr,
-                                       vout.ptr, None, stream_ptr()),
-              'Operator.apply')
+        ph = None
+        if vin.code == GPD_C128 and phase_cd is not None:
+            ph = np.ascontiguousarray(np.asarray(phase_cd, np.complex128)
+                                      .reshape(3, 2)).view(np.float64)
+        check(lib().gpd_operator_apply(
+            self._handle, vin.code, nin, vin.ptr, vout.ptr,
+            None if ph is None else ph.ctypes.data, stream_ptr()),
+            'Operator.apply')
         vout.finish()
 
     def set_launch(self, wg_target):

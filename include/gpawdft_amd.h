@@ -29,7 +29,7 @@ enum {
     GPD_ERR_NOMEM = -4
 };
 
-enum { GPD_F64 = 0, GPD_F32 = 1 };
+enum { GPD_F64 = 0, GPD_F32 = 1, GPD_C128 = 2 };
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* gpd_last_error(void);
@@ -53,7 +53,13 @@ int gpd_operator_create(const double* coefs, const long* offsets, int ncoefs,
                         gpd_operator** out);
 
 /* Operator.apply(in, out, phases) (c/operators.c:316-405): out = A in for
- * nin grids.  phases must be NULL.                                    */
+ * nin grids.  For GPD_F64 / GPD_F32 phases must be NULL.  For GPD_C128
+ * (interleaved re, im) phases is a host array of 12 doubles, the (3, 2)
+ * complex phase_cd: [axis][0] multiplies the periodic image above the
+ * cell, [axis][1] the image below (c/bc.c:227-247 with
+ * c/bmgs/translate.c:27-47), a corner carrying axis 0's phase first; NULL
+ * means all ones (Gamma).  Bit-compatible with the reference's complex
+ * Operator in every dtype.                                             */
 int gpd_operator_apply(gpd_operator* op, int dtype, long nin,
                        const void* d_in, void* d_out, const void* phases,
                        void* stream);

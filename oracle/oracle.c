@@ -16,6 +16,9 @@
  * Parity status (how each function is pinned):
  *   orc_halo_fill, orc_fd     -- reference Operator (oracle/_ref) bit-for-bit,
  *                                FD symbol closed form, polynomial exactness.
+ *   orc_halo_fill_z, orc_fdz  -- reference complex Operator with phases
+ *                                (oracle/_ref) bit-for-bit; Bloch plane waves
+ *                                are eigenvectors with the FD symbol at k+G.
  *   orc_restrict/interpolate  -- reference Transformer bit-for-bit, integral
  *                                conservation, polynomial reproduction.
  *   orc_spline_*              -- reference Spline bit-for-bit, interpolation
@@ -105,6 +108,128 @@ void orc_halo_fill(const double* a, const long n[3], const long lo[3],
             end[i] = m[i];
         }
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* Complex halo fill with Bloch phases.  As orc_halo_fill, on interleaved
+ * (re, im) values, with the periodic copies multiplied by the phase of
+ * their direction (c/bc.c:227-247 with c/bmgs/translate.c:27-47,
+ * translatemz): the right pad of axis i receives phase_cd[i][0] times the
+ * first planes, the left pad phase_cd[i][1] times the last planes.  The
+ * product is the C99 complex product of the reference's build:
+ * re = pr*sr - pi*si, im = pr*si + pi*sr, two roundings each.  With
+ * `corners`, axis i copies the pads of the axes before it, so a corner
+ * value carries the phases of both axes, axis 0's applied first.
+ * phase: 12 doubles, [axis][direction][re, im].                         */
+void orc_halo_fill_z(const double* a, const long n[3], const long lo[3],
+                     const long hi[3], const int pbc[3], int corners,
+                     const double* phase, double* b)
+{
+    long m[3];
+    for (int c = 0; c < 3; c++) m[c] = n[c] + lo[c] + hi[c];
+    memset(b, 0, sizeof(double) * 2 * m[0] * m[1] * m[2]);
+    for (long i0 = 0; i0 < n[0]; i0++)
+        for (long i1 = 0; i1 < n[1]; i1++)
+            memcpy(b + 2 * (((i0 + lo[0]) * m[1] + i1 + lo[1]) * m[2] + lo[2]),
+                   a + 2 * ((i0 * n[1] + i1) * n[2]), sizeof(double) * 2 * n[2]);
+
+    long beg[3] = {lo[0], lo[1], lo[2]};
+    long end[3] = {lo[0] + n[0], lo[1] + n[1], lo[2] + n[2]};
+    for (int i = 0; i < 3; i++) {
+        if (pbc[i]) {
+            for (int side = 0; side < 2; side++) {
+                long d0, d1, s0;
+                if (side == 0) {          /* right pad <- first planes */
+                    d0 = lo[i] + n[i];
+                    d1 = d0 + hi[i];
+                    s0 = lo[i];
+                } else {                  /* left pad <- last planes */
+                    d0 = 0;
+                    d1 = lo[i];
+                    s0 = lo[i] + n[i] - lo[i];
+                }
+                const double pr = phase[4 * i + 2 * side];
+                const double pim = phase[4 * i + 2 * side + 1];
+                long r0[3] = {beg[0], beg[1], beg[2]};
+                long r1[3] = {end[0], end[1], end[2]};
+                r0[i] = d0;
+                r1[i] = d1;
+                for (long p0 = r0[0]; p0 < r1[0]; p0++)
+                    for (long p1 = r0[1]; p1 < r1[1]; p1++)
+                        for (long p2 = r0[2]; p2 < r1[2]; p2++) {
+                            long q[3] = {p0, p1, p2};
+                            q[i] = s0 + (q[i] - d0);
+                            const double* src = b + 2 * ((q[0] * m[1] + q[1]) * m[2] + q[2]);
+                            double* dst = b + 2 * ((p0 * m[1] + p1) * m[2] + p2);
+                            const double sr = src[0], si = src[1];
+                            const double re1 = pr * sr, re2 = pim * si;
+                            const double im1 = pr * si, im2 = pim * sr;
+                            dst[0] = re1 - re2;
+                            dst[1] = im1 + im2;
+                        }
+            }
+        }
+        if (corners) {
+            beg[i] = 0;
+            end[i] = m[i];
+        }
+    }
+}
+
+/* Complex FD stencil (c/bmgs/fd.c:16-87 built with T = double_complex,
+ * c/operators.c:109-157 with phases): per band the phased halo, then per
+ * point x += a[p + off[c]] * coefs[c] from 0 in coefficient order -- a
+ * real coefficient times a complex value is (re * c, im * c), so each
+ * part is summed exactly as the real stencil sums it.  in/out hold
+ * interleaved (re, im) values.                                         */
+void orc_fdz(int ncoefs, const double* coefs, const long* off_c,
+             const long n[3], const int pbc[3], const double* phase, long nb,
+             const double* in, double* out)
+{
+    long r = 0;
+    int corners = 0;
+    for (int c = 0; c < ncoefs; c++) {
+        int nonzero = 0;
+        for (int k = 0; k < 3; k++) {
+            long d = off_c[3 * c + k];
+            if (d > r) r = d;
+            if (d != 0) nonzero++;
+        }
+        if (nonzero >= 2) corners = 1;
+    }
+    long lo[3] = {r, r, r}, hi[3] = {r, r, r};
+    long m[3] = {n[0] + 2 * r, n[1] + 2 * r, n[2] + 2 * r};
+    long ng = n[0] * n[1] * n[2];
+    long* flat = (long*)malloc(sizeof(long) * ncoefs);
+    for (int c = 0; c < ncoefs; c++)
+        flat[c] = (off_c[3 * c] * m[1] + off_c[3 * c + 1]) * m[2] +
+                  off_c[3 * c + 2];
+#pragma omp parallel
+    {
+        double* buf = (double*)malloc(sizeof(double) * 2 * m[0] * m[1] * m[2]);
+#pragma omp for schedule(static)
+        for (long x = 0; x < nb; x++) {
+            orc_halo_fill_z(in + 2 * x * ng, n, lo, hi, pbc, corners, phase, buf);
+            double* o = out + 2 * x * ng;
+            for (long i0 = 0; i0 < n[0]; i0++)
+                for (long i1 = 0; i1 < n[1]; i1++) {
+                    const double* a =
+                        buf + 2 * (((i0 + r) * m[1] + i1 + r) * m[2] + r);
+                    for (long i2 = 0; i2 < n[2]; i2++) {
+                        double sr = 0.0, si = 0.0;
+                        for (int c = 0; c < ncoefs; c++) {
+                            const double* v = a + 2 * (i2 + flat[c]);
+                            sr += v[0] * coefs[c];
+                            si += v[1] * coefs[c];
+                        }
+                        *o++ = sr;
+                        *o++ = si;
+                    }
+                }
+        }
+        free(buf);
+    }
+    free(flat);
 }
 
 /* ------------------------------------------------------------------ */

@@ -38,6 +38,8 @@ def lib():
         sig = {
             'orc_halo_fill': (None, [_dp, _lp, _lp, _lp, _ip, i, _dp]),
             'orc_fd': (None, [i, _dp, _lp, _lp, _ip, l, _dp, _dp]),
+            'orc_fdz': (None, [i, _dp, _lp, _lp, _ip, _dp, l, _dp, _dp]),
+            'orc_halo_fill_z': (None, [_dp, _lp, _lp, _lp, _ip, i, _dp, _dp]),
             'orc_relax': (None, [i, i, _dp, _lp, _lp, _ip, l, _dp, _dp, i,
                                  d]),
             'orc_transform': (None, [i, i, _lp, _lp, _lp, _ip, _ip, _lp, l,
@@ -105,6 +107,25 @@ def fd(coefs, offsets_c, pbc_c, in_xg):
     pbc = _i32(np.asarray(pbc_c, bool).astype(int))
     lib().orc_fd(len(coefs), _d(coefs), _l(off), _l(n), _i(pbc), nb,
                  _d(in_xg), _d(out))
+    return out
+
+
+def fdz(coefs, offsets_c, pbc_c, phase_cd, in_xg):
+    """Complex FD operator apply with Bloch phases (c/operators.c:109-157
+    with phases, c/bc.c:227-247, c/bmgs/fd.c:16-87 for double_complex).
+    phase_cd: (3, 2) complex, [axis][0] for the image above, [axis][1]
+    below."""
+    in_xg = np.ascontiguousarray(in_xg, np.complex128)
+    n = _i64(in_xg.shape[-3:])
+    nb = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
+    out = np.empty_like(in_xg)
+    coefs = _f64(coefs)
+    off = _i64(np.asarray(offsets_c).reshape(-1, 3))
+    pbc = _i32(np.asarray(pbc_c, bool).astype(int))
+    ph = np.ascontiguousarray(np.asarray(phase_cd, np.complex128)
+                              .reshape(3, 2)).view(np.float64)
+    lib().orc_fdz(len(coefs), _d(coefs), _l(off), _l(n), _i(pbc), _d(ph), nb,
+                  _d(in_xg.view(np.float64)), _d(out.view(np.float64)))
     return out
 
 

@@ -49,7 +49,7 @@ def grid(N, pbc, h=0.2, cell=None):
     return GridDescriptor(N, N * h / Bohr if cell is None else cell, pbc)
 
 
-def ref_operator(R, gd, coefs, offs):
+def ref_operator(R, gd, coefs, offs, real=1):
     offs = np.asarray(offs, int).reshape(-1, 3)
     cfd = not any((o != 0).sum() >= 2 for o in offs)
     mp = int(offs.max())
@@ -58,8 +58,8 @@ def ref_operator(R, gd, coefs, offs):
     return R.Operator(np.ascontiguousarray(coefs, float),
                       np.ascontiguousarray(offs @ stride, np.int64),
                       np.ascontiguousarray(gd.n_c, np.int64), mp,
-                      np.ascontiguousarray(gd.neighbor_cd, np.int64), 1, None,
-                      int(cfd))
+                      np.ascontiguousarray(gd.neighbor_cd, np.int64), real,
+                      None, int(cfd))
 
 
 def ref_transformer(R, gdin, gdout, nn):
@@ -140,6 +140,34 @@ def main():
                        'offsets': np.asarray(o, int).tolist(),
                        'fd_sha256': digest(apply(ref_operator(R, gd, c, o),
                                                  a, gd.n_c))}
+
+    # complex (Bloch) stencil: phases exp(2 pi i sdisp_cd k_c) with the
+    # single-domain sdisp_cd = [[1, -1]] * 3 (gpaw/domain.py:122-136,
+    # gpaw/kpt_descriptor.py:310-313); the hexagonal cell has off-axis
+    # points, so corner pads carry two phases
+    zcases = []
+    for N, pbc, nn, k_c, zcell, seed in (
+            ((12, 10, 14), (1, 1, 1), 3, (0.25, -0.1, 0.4), None, 400),
+            ((10, 12, 8), (1, 0, 1), 2, (0.5, 0.0, -0.3), None, 401),
+            ((20, 20, 24), (1, 1, 1), 2, (1 / 3, -1 / 3, 0.2), cell, 402)):
+        if zcell is None:
+            gd = grid(N, pbc)
+        else:
+            gd = GridDescriptor(N, zcell, pbc)
+        c, o, _ = guc_laplace_stencil(gd, -0.5, nn)
+        phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                       np.array(k_c)[:, None])
+        rng = np.random.default_rng(seed)
+        shape = (2,) + tuple(int(x) for x in gd.n_c)
+        a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        out = np.empty_like(a)
+        ref_operator(R, gd, c, o, real=0).apply(a, out, phase)
+        zcases.append({'N': list(N), 'pbc': list(pbc), 'nn': nn,
+                       'k_c': list(k_c),
+                       'cell': None if zcell is None else zcell.tolist(),
+                       'seed': seed,
+                       'fdz_sha256': digest(out.view(np.float64))})
+    fx['complex_fd'] = zcases
 
     # splines and spline_to_grid
     rc, rmax = 1.2 / Bohr, 2.5 / Bohr

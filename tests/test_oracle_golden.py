@@ -186,3 +186,29 @@ def test_poisson_vcycles_match_reference(case):
     for it in range(px['vcycles']):
         ps.iterate2(phis, rhos)
     assert digest(phis[0]) == px['phi_sha256']
+
+
+def _complex_case(case):
+    if case['cell'] is None:
+        gd = grid(case['N'], tuple(case['pbc']))
+    else:
+        gd = GridDescriptor(case['N'], np.array(case['cell']),
+                            tuple(case['pbc']))
+    c, o, _ = guc_laplace_stencil(gd, -0.5, case['nn'])
+    phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                   np.array(case['k_c'])[:, None])
+    rng = np.random.default_rng(case['seed'])
+    shape = (2,) + tuple(int(x) for x in gd.n_c)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    return gd, c, o, phase, a
+
+
+@pytest.mark.parametrize('case', range(3))
+def test_complex_stencil_matches_reference(case):
+    """Complex Operator.apply with Bloch phases (c/operators.c:109-157,
+    c/bc.c:227-247): periodic, mixed-boundary and hexagonal (corner pads
+    with two phases) cells, bit for bit with the reference's C."""
+    cx = FX['complex_fd'][case]
+    gd, c, o, phase, a = _complex_case(cx)
+    out = O.fdz(c, o, gd.pbc_c, phase, a)
+    assert digest(out.view(np.float64)) == cx['fdz_sha256']

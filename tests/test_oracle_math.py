@@ -354,3 +354,34 @@ def test_davidson_restatement_converges_to_dense_spectrum():
     assert np.abs(eps[:5] - w[:5]).max() < 1e-9
     S = sysm.gd.dv * psi @ psi.T
     assert np.abs(S - np.eye(8)).max() < 1e-10
+
+
+def test_complex_stencil_bloch_plane_waves():
+    """With phases exp(2 pi i k_c) for the image above and exp(-2 pi i k_c)
+    below, a plane wave exp(i (k + G) . r) sampled on a periodic grid is an
+    eigenvector of the FD operator with eigenvalue
+    sum_c coef_c exp(i (k + G) . h off_c): checks the phase convention and
+    direction independently of the reference (orthorhombic and hexagonal
+    cells)."""
+    import oracle as O
+    from gpawdft_amd.fd_operators import guc_laplace_stencil
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    for N, cell, nn in (((12, 10, 14), np.diag([4.0, 3.5, 5.0]), 3),
+                        ((10, 10, 12), np.array([[4.0, 0, 0],
+                                                 [2.0, 3.4641016, 0],
+                                                 [0, 0, 5.0]]), 2)):
+        gd = GridDescriptor(N, cell, True)
+        c, o, _ = guc_laplace_stencil(gd, -0.5, nn)
+        k_c = np.array([0.25, -0.1, 0.4])
+        G_c = np.array([1, -2, 3])
+        phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) * k_c[:, None])
+        idx = np.indices(N).reshape(3, -1).T            # grid indices
+        frac = idx / np.array(N)                        # scaled positions
+        psi = np.exp(2j * np.pi * frac @ (k_c + G_c)).reshape(N)
+        out = O.fdz(c, o, gd.pbc_c, phase, psi[None])[0]
+        off = np.asarray(o).reshape(-1, 3) / np.array(N)
+        lam = np.sum(np.asarray(c) * np.exp(2j * np.pi * off @ (k_c + G_c)))
+        assert np.abs(out - lam * psi).max() < 1e-11 * np.abs(lam)
+        # the wrong direction convention (phases swapped) breaks it
+        bad = O.fdz(c, o, gd.pbc_c, phase[:, ::-1], psi[None])[0]
+        assert np.abs(bad - lam * psi).max() > 1e-3 * np.abs(lam)

@@ -92,70 +92,6 @@ __global__ __launch_bounds__(64) void lfc_integrate_seq_kernel(
     *cp = acc;
 }
 
-// FP64-exact integrate, one pass per key: a thread holds the chains of up
-// to NMT functions of one key (one band), walks the key's point list once
-// and adds each point into every chain in the reference's order, so psi is
-// read once per key instead of once per function.  The key (blockIdx.y)
-// and the function slice (blockIdx.z) are uniform over the workgroup; the
-// lanes are bands.
-template <typename T, int NMT>
-__global__ __launch_bounds__(64) void lfc_integrate_key_kernel(
-    const T* __restrict__ a, T* __restrict__ c, long nG, int Mmax, long nx, T dv,
-    const T* __restrict__ A, const int* __restrict__ key_M, const int* __restrict__ key_nm,
-    const long* __restrict__ key_off, const int* __restrict__ key_G,
-    const int* __restrict__ key_row)
-{
-    constexpr int BATCH = 16;
-    const int k = blockIdx.y;
-    const int m0 = blockIdx.z * NMT;
-    const int nm = key_nm[k];
-    if (m0 >= nm) return;
-    const int nmt = min(NMT, nm - m0);
-    const long band = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (band >= nx) return;
-    const T* ab = a + band * nG;
-    T* cp = c + band * Mmax + key_M[k] + m0;
-    T acc[NMT];
-#pragma unroll
-    for (int m = 0; m < NMT; m++) acc[m] = m < nmt ? cp[m] : T(0);
-    const long j0 = key_off[k], j1 = key_off[k + 1];
-
-    T xa[BATCH], xb[BATCH];
-    auto load = [&](long j, T* xv) {
-#pragma unroll
-        for (int q = 0; q < BATCH; q++)
-            if (j + q < j1) xv[q] = ab[key_G[j + q]];
-    };
-    auto add = [&](long j, const T* xv) {
-#pragma unroll
-        for (int q = 0; q < BATCH; q++) {
-            if (j + q < j1) {
-                const T xd = xv[q] * dv;
-                const T* Ar = A + (long)key_row[j + q] + m0;
-#pragma unroll
-                for (int m = 0; m < NMT; m++)
-                    if (m < nmt) acc[m] = madd<T>(acc[m], xd, Ar[m]);
-            }
-        }
-    };
-    long j = j0;
-    if (j < j1) load(j, xa);
-    while (j < j1) {
-        const long jn = j + BATCH;
-        if (jn < j1) load(jn, xb);
-        add(j, xa);
-        j = jn;
-        if (j >= j1) break;
-        const long jm = j + BATCH;
-        if (jm < j1) load(jm, xa);
-        add(j, xb);
-        j = jm;
-    }
-#pragma unroll
-    for (int m = 0; m < NMT; m++)
-        if (m < nmt) cp[m] = acc[m];
-}
-
 // Grouped integrate: one workgroup per (group, block of NBB bands).  NF is
 // the largest function count of a group the instantiation holds in
 // registers (9: one atom with l = 0..2; 16 otherwise).
@@ -635,14 +571,8 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
 
     // function M -> its key (-1 for an index no volume carries)
     std::vector<int> fkey(std::max(Mmax, 1), -1);
-    l->keys_disjoint = true;
-    for (int k = 0; k < l->nkeys; k++) {
-        l->max_nm = std::max(l->max_nm, h.key_nm[k]);
-        for (int m = 0; m < h.key_nm[k]; m++) {
-            if (fkey[h.key_M[k] + m] >= 0) l->keys_disjoint = false;
-            fkey[h.key_M[k] + m] = k;
-        }
-    }
+    for (int k = 0; k < l->nkeys; k++)
+        for (int m = 0; m < h.key_nm[k]; m++) fkey[h.key_M[k] + m] = k;
 
     // per-key flat arrays
     std::vector<long> key_off(l->nkeys + 1, 0);
@@ -724,16 +654,6 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
     GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
     hipStream_t st = (hipStream_t)stream;
     if (dtype == GPD_F64) {
-        if (l->keys_disjoint && l->max_nm <= 9 && l->nkeys <= 65535) {
-            constexpr int NMT = 9;
-            dim3 grid((unsigned)((nx + 63) / 64), (unsigned)l->nkeys,
-                      (unsigned)((l->max_nm + NMT - 1) / NMT));
-            hipLaunchKernelGGL((lfc_integrate_key_kernel<double, NMT>), grid, dim3(64), 0, st,
-                               (const double*)d_a_xG, (double*)d_c_xM, l->nG, l->Mmax, nx,
-                               l->dv, l->A64, l->key_M, l->key_nm, l->key_off, l->key_G,
-                               l->key_row);
-            return last_error_to_status(hipGetLastError());
-        }
         long n = nx * l->Mmax;
         hipLaunchKernelGGL(lfc_integrate_seq_kernel<double>, dim3((n + 63) / 64), dim3(64), 0, st,
                            (const double*)d_a_xG, (double*)d_c_xM, l->nG, l->Mmax, nx, l->dv,

@@ -62,8 +62,6 @@ struct gpd_lfc {
     int* key_G = nullptr;
     int* key_row = nullptr;
     int* fun_key = nullptr;    // Mmax: key of each function index
-    int max_nm = 0;            // largest key_nm
-    bool keys_disjoint = false;   // key M ranges do not overlap
     LfcGroups gi;              // grouped integrate (FP32), from the keys
     LfcGroups ga;              // grouped add (FP32), from the sub-keys
     bool add_shares_groups = true;   // no repeated points: ga aliases gi

@@ -253,15 +253,93 @@ __global__ void lfc_add_kernel(const T* __restrict__ c, T* __restrict__ a, long 
 }
 
 // ------------------------------------------------------------------ //
+// complex (Bloch) functions, exact order (c/lfc.c:400-423, 758-781): a
+// volume's values are multiplied by its phase for this k-point.
+
+// integrate: one thread per (band, function M), bands fastest, the chain
+// of the FP64 integrate with c += (a * (phase * dv)) * A: phase * dv is
+// (pr * dv, pi * dv), a * vphase the C99 complex product, then each part
+// times A and added.
+__global__ __launch_bounds__(64) void lfc_integrate_z_kernel(
+    const double2* __restrict__ a, double2* __restrict__ c, long nG, int Mmax, long nx,
+    double dv, const double* __restrict__ A, const int* __restrict__ fun_key,
+    const int* __restrict__ key_M, const long* __restrict__ key_off,
+    const int* __restrict__ key_G, const int* __restrict__ key_row,
+    const int* __restrict__ key_W, const double2* __restrict__ phase_W)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * Mmax) return;
+    const long band = idx % nx;
+    const int Mi = (int)(idx / nx);
+    const int k = fun_key[Mi];
+    if (k < 0) return;
+    const int m = Mi - key_M[k];
+    const double2* ab = a + band * nG;
+    double2* cp = c + band * Mmax + Mi;
+    double2 acc = *cp;
+    const long j1 = key_off[k + 1];
+    for (long j = key_off[k]; j < j1; j++) {
+        const double2 x = ab[key_G[j]];
+        const double2 ph = phase_W[key_W[j]];
+        const double vr = ph.x * dv, vi = ph.y * dv;
+        const double r1 = x.x * vr, r2 = x.y * vi;
+        const double i1 = x.x * vi, i2 = x.y * vr;
+        const double avr = r1 - r2, avi = i1 + i2;
+        const double Am = A[(long)key_row[j] + m];
+        acc.x = madd<double>(acc.x, avr, Am);
+        acc.y = madd<double>(acc.y, avi, Am);
+    }
+    *cp = acc;
+}
+
+// add: one thread per (covered point, band); per entry in the sweep's
+// volume order t = sum_m A c (from 0), then a += t * conj(phase).
+__global__ void lfc_add_z_kernel(const double2* __restrict__ c, double2* __restrict__ a,
+                                 long nG, int Mmax, long nx, long ncov,
+                                 const double* __restrict__ A, const int* __restrict__ cov_G,
+                                 const int* __restrict__ cov_ptr, const int2* __restrict__ ent,
+                                 const int* __restrict__ ent_W,
+                                 const double2* __restrict__ phase_W)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * ncov) return;
+    const long i = idx % ncov;
+    const long b = idx / ncov;
+    const long G = cov_G[i];
+    double2 v = a[b * nG + G];
+    const double2* cb = c + b * Mmax;
+    const int e1 = cov_ptr[i + 1];
+    for (int e = cov_ptr[i]; e < e1; e++) {
+        const int2 en = ent[e];
+        const int nm = en.y & 15;
+        const double* Ar = A + en.x;
+        const double2* cm = cb + (en.y >> 4);
+        double tr = 0.0, ti = 0.0;
+        for (int m = 0; m < nm; m++) {
+            tr = madd<double>(tr, Ar[m], cm[m].x);
+            ti = madd<double>(ti, Ar[m], cm[m].y);
+        }
+        const double2 ph = phase_W[ent_W[e]];
+        const double pr = ph.x, pim = -ph.y;
+        const double r1 = tr * pr, r2 = ti * pim;
+        const double i1 = tr * pim, i2 = ti * pr;
+        v.x = v.x + (r1 - r2);
+        v.y = v.y + (i1 + i2);
+    }
+    a[b * nG + G] = v;
+}
+
+// ------------------------------------------------------------------ //
 // host-side sweep
 
 struct HostTables {
     std::vector<int> ptr;            // nG + 1
     std::vector<int2> ent;
+    std::vector<int> ent_W;          // volume of each entry
     std::vector<int> cov_G, cov_ptr;
     std::map<int, int> key_of_M;     // M -> key
     std::vector<int> key_M, key_nm;
-    std::vector<std::vector<int>> key_G, key_row;
+    std::vector<std::vector<int>> key_G, key_row, key_W;
 };
 
 static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_W,
@@ -295,6 +373,7 @@ static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_
     }
     for (long G = 0; G < nG; G++) h.ptr[G + 1] = h.ptr[G] + count[G];
     h.ent.resize(h.ptr[nG]);
+    h.ent_W.resize(h.ptr[nG]);
     std::vector<int> fill(h.ptr.begin(), h.ptr.end() - 1);
     for (int W = 0; W < nW; W++) {
         auto it = h.key_of_M.find(M_W[W]);
@@ -309,6 +388,7 @@ static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_
     }
     h.key_G.assign(h.key_M.size(), {});
     h.key_row.assign(h.key_M.size(), {});
+    h.key_W.assign(h.key_M.size(), {});
 
     long Ga = 0;
     for (int B = 0; B < nB; B++) {
@@ -322,6 +402,7 @@ static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_
                     int nm = nm_W[v.W];
                     long rowoff = A_off[v.W] + (v.row + (G - Ga)) * nm;
                     if (rowoff > 0x7fffffffL) return false;
+                    h.ent_W[fill[G]] = v.W;
                     h.ent[fill[G]++] = make_int2((int)rowoff, (M_W[v.W] << 4) | nm);
                 }
             for (const Vol& v : vol_i) {
@@ -330,6 +411,7 @@ static bool run_sweep(int nW, const long* A_off, const int* ng_W, const int* nm_
                 for (long G = Ga; G < Gb; G++) {
                     h.key_G[k].push_back((int)G);
                     h.key_row[k].push_back((int)(A_off[v.W] + (v.row + (G - Ga)) * nm));
+                    h.key_W[k].push_back(v.W);
                 }
             }
             for (Vol& v : vol_i) v.row += nGs;
@@ -530,7 +612,8 @@ extern "C" int gpd_lfc_destroy(gpd_lfc* l)
 {
     if (!l) return GPD_OK;
     void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
-                    l->key_off, l->key_G, l->key_row, l->fun_key};
+                    l->key_off, l->key_G, l->key_row, l->fun_key, l->ent_W, l->key_W,
+                    l->phase_W};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_groups(l->gi);
@@ -577,13 +660,17 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     // per-key flat arrays
     std::vector<long> key_off(l->nkeys + 1, 0);
     for (int k = 0; k < l->nkeys; k++) key_off[k + 1] = key_off[k] + (long)h.key_G[k].size();
-    std::vector<int> kG, kR;
+    std::vector<int> kG, kR, kW;
     kG.reserve(key_off[l->nkeys]);
     kR.reserve(key_off[l->nkeys]);
+    kW.reserve(key_off[l->nkeys]);
     for (int k = 0; k < l->nkeys; k++) {
         kG.insert(kG.end(), h.key_G[k].begin(), h.key_G[k].end());
         kR.insert(kR.end(), h.key_row[k].begin(), h.key_row[k].end());
+        kW.insert(kW.end(), h.key_W[k].begin(), h.key_W[k].end());
     }
+    l->nW = nW;
+    std::vector<double2> ones(std::max(nW, 1), make_double2(1.0, 0.0));
 
     hipError_t e = build_groups(h.key_M, h.key_nm, h.key_G, h.key_row, A, nG, l->gi);
     // sub-keys for the grouped add: the j-th visit of a point in a key's
@@ -635,6 +722,9 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     UP(key_G, kG.data(), kG.size());
     UP(key_row, kR.data(), kR.size());
     UP(fun_key, fkey.data(), fkey.size());
+    UP(ent_W, h.ent_W.data(), h.ent_W.size());
+    UP(key_W, kW.data(), kW.size());
+    UP(phase_W, ones.data(), ones.size());
 #undef UP
     if (e != hipSuccess) {
         set_last_error(hipGetErrorString(e));
@@ -681,8 +771,32 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
         }
         return last_error_to_status(hipGetLastError());
     }
-
+    if (dtype == GPD_C128) {
+        long n = nx * l->Mmax;
+        hipLaunchKernelGGL(lfc_integrate_z_kernel, dim3((n + 63) / 64), dim3(64), 0, st,
+                           (const double2*)d_a_xG, (double2*)d_c_xM, l->nG, l->Mmax, nx, l->dv,
+                           l->A64, l->fun_key, l->key_M, l->key_off, l->key_G, l->key_row,
+                           l->key_W, l->phase_W);
+        return last_error_to_status(hipGetLastError());
+    }
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
+}
+
+extern "C" int gpd_lfc_set_phases(gpd_lfc* l, const double* phase_W)
+{
+    GPD_REQUIRE(l, GPD_ERR_ARG, "null lfc");
+    if (l->nW == 0) return GPD_OK;
+    std::vector<double2> ph(l->nW);
+    for (int W = 0; W < l->nW; W++)
+        ph[W] = phase_W ? make_double2(phase_W[2 * W], phase_W[2 * W + 1])
+                        : make_double2(1.0, 0.0);
+    hipError_t e = hipMemcpy(l->phase_W, ph.data(), sizeof(double2) * l->nW,
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_last_error(hipGetErrorString(e));
+        return GPD_ERR_HIP;
+    }
+    return GPD_OK;
 }
 
 extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, void* d_a_xG,
@@ -696,5 +810,12 @@ extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, v
     if (dtype == GPD_F64) return lfc_add_exact_f64(l, nx, (const double*)d_c_xM, (double*)d_a_xG, st);
     if (dtype == GPD_F32)
         return lfc_add_grouped_f32(l, nx, (const float*)d_c_xM, (float*)d_a_xG, st);
+    if (dtype == GPD_C128) {
+        const long n = nx * l->ncov;
+        hipLaunchKernelGGL(lfc_add_z_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+                           (const double2*)d_c_xM, (double2*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
+                           l->A64, l->cov_G, l->cov_ptr, l->ent, l->ent_W, l->phase_W);
+        return last_error_to_status(hipGetLastError());
+    }
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
 }

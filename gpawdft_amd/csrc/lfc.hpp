@@ -62,6 +62,12 @@ struct gpd_lfc {
     int* key_G = nullptr;
     int* key_row = nullptr;
     int* fun_key = nullptr;    // Mmax: key of each function index
+    // complex (Bloch) path: the volume of each point entry and of each key
+    // sequence element, and the current phase of every volume
+    int nW = 0;
+    int* ent_W = nullptr;      // nent
+    int* key_W = nullptr;      // key_off[nkeys]
+    double2* phase_W = nullptr;   // nW (all ones until set)
     LfcGroups gi;              // grouped integrate (FP32), from the keys
     LfcGroups ga;              // grouped add (FP32), from the sub-keys
     bool add_shares_groups = true;   // no repeated points: ga aliases gi

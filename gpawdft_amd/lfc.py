@@ -96,9 +96,6 @@ class Sphere:
 class NewLocalizedFunctionsCollection:
     def __init__(self, gd, spline_aj, kd=None, cut=False, dtype=float,
                  integral=None, forces=None):
-        if kd is not None and not getattr(kd, 'gamma', True):
-            raise NotImplementedError('k-points (Bloch phases) are not '
-                                      'supported')
         if integral is not None:
             raise NotImplementedError('normalised LFCs are not part of this '
                                       'build')
@@ -107,7 +104,13 @@ class NewLocalizedFunctionsCollection:
         self.cut = cut
         self.dtype = dtype
         self.Mmax = None
-        self.gamma = True
+        # k-points: kd needs .gamma and .ibzk_qc (scaled k-vectors of this
+        # rank's k-points); complex functions then carry the phases
+        # exp(2 pi i k_q . sdisp_W) of their periodic images
+        self.gamma = kd is None or getattr(kd, 'gamma', True)
+        self.ibzk_qc = None if self.gamma else np.asarray(kd.ibzk_qc, float)
+        self.phase_qW = None
+        self._q_set = None
         self.use_global_indices = False
         self.my_atom_indices = None
         self._handle = None
@@ -118,6 +121,7 @@ class NewLocalizedFunctionsCollection:
     def _destroy(self, _release=_lib.release):
         _release('gpd_lfc_destroy', getattr(self, '_handle', None))
         self._handle = None
+        self._q_set = None
 
     def get_function_count(self, a):
         return self.sphere_a[a].get_function_count()
@@ -183,10 +187,37 @@ class NewLocalizedFunctionsCollection:
         self.G_B = np.ascontiguousarray(self.G_B[indices])
         self.W_B = np.ascontiguousarray(self.W_B[indices])
 
+        # gpaw/lfc.py:336-343
+        if self.gamma:
+            if self.dtype == float:
+                self.phase_qW = np.empty((0, nW), complex)
+            else:
+                self.phase_qW = np.ones((1, nW), complex)
+        else:
+            self.phase_qW = np.exp(2j * np.pi *
+                                   np.inner(self.ibzk_qc, sdisp_Wc))
+        self.sdisp_Wc = sdisp_Wc
+
         self._destroy()
         for a in self.atom_indices:
             self.sphere_a[a].ranks = np.array([], int)
         return sdisp_Wc
+
+    def _select_q(self, code, q):
+        """Upload the phases of k-point q for complex arrays."""
+        if code != _lib.GPD_C128:
+            return
+        if len(self.phase_qW) == 0:
+            raise ValueError('complex arrays need dtype=complex or k-points')
+        q = 0 if (q < 0 and len(self.phase_qW) == 1) else q
+        if q < 0:
+            raise ValueError('a k-point index q is needed')
+        if self._q_set != q:
+            ph = np.ascontiguousarray(self.phase_qW[q], np.complex128)
+            check(lib().gpd_lfc_set_phases(
+                self.handle, _lib.ptr_double(ph.view(np.float64))),
+                'LFC phases')
+            self._q_set = q
 
     def host_tables(self):
         """(A concatenated, ng_W, nm_W) of the current volumes."""
@@ -257,6 +288,7 @@ class NewLocalizedFunctionsCollection:
                               device=va.t.device)
         else:
             c_xM = self._pack(c_axi, xshape, va.t)
+        self._select_q(va.code, q)
         check(lib().gpd_lfc_add(self.handle, va.code, nx, c_xM.data_ptr(),
                                 va.ptr, stream_ptr()), 'LFC.add')
         va.finish()
@@ -269,7 +301,7 @@ class NewLocalizedFunctionsCollection:
         nx = int(np.prod(xshape, dtype=np.int64))
         c_xM = torch.zeros((nx, self.Mmax), dtype=va.t.dtype,
                            device=va.t.device)
-        self.integrate_xM(va, c_xM)
+        self.integrate_xM(va, c_xM, q)
         for a in self.atom_indices:
             if a not in c_axi:
                 continue
@@ -281,19 +313,21 @@ class NewLocalizedFunctionsCollection:
             else:
                 dst.copy_(src)
 
-    def integrate_xM(self, view_or_tensor, c_xM):
+    def integrate_xM(self, view_or_tensor, c_xM, q=-1):
         """Low-level: c_xM (nx, Mmax device tensor) += integrals."""
         va = (view_or_tensor if isinstance(view_or_tensor, DeviceView)
               else DeviceView(view_or_tensor))
         nx = int(np.prod(va.t.shape[:-3], dtype=np.int64))
+        self._select_q(va.code, q)
         check(lib().gpd_lfc_integrate(self.handle, va.code, nx, va.ptr,
                                       c_xM.data_ptr(), stream_ptr()),
               'LFC.integrate')
 
-    def add_xM(self, c_xM, a_xG):
+    def add_xM(self, c_xM, a_xG, q=-1):
         """Low-level: a_xG += sum_M c_xM A_GM (device tensors)."""
         va = DeviceView(a_xG)
         nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+        self._select_q(va.code, q)
         check(lib().gpd_lfc_add(self.handle, va.code, nx, c_xM.data_ptr(),
                                 va.ptr, stream_ptr()), 'LFC.add')
 

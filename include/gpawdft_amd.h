@@ -132,8 +132,9 @@ int gpd_spline_to_grid(const gpd_spline* s, const long beg[3],
 
 /* ------------------------------------------------------------------ */
 /* Localized functions (PAW projectors) on the grid.
- * Replaces _gpaw.LFC(A_Wgm, M_W, G_B, W_B, dv, phase_qW) (c/lfc.c:106-186)
- * for real (Gamma-point) functions.  Volume W has ng_W[W] rows of nm_W[W]
+ * Replaces _gpaw.LFC(A_Wgm, M_W, G_B, W_B, dv, phase_qW) (c/lfc.c:106-186);
+ * the phases of complex functions are set per k-point by
+ * gpd_lfc_set_phases.  Volume W has ng_W[W] rows of nm_W[W]
  * values, concatenated row-major in A (host, FP64); (G_B, W_B) is the
  * boundary list sorted by G (gpaw/lfc.py:276-350).  nG is the number of
  * points of one grid and Mmax the number of functions.  The object keeps
@@ -154,6 +155,14 @@ int gpd_lfc_integrate(gpd_lfc* lfc, int dtype, long nx, const void* d_a_xG,
  * reference's volume order.                                            */
 int gpd_lfc_add(gpd_lfc* lfc, int dtype, long nx, const void* d_c_xM,
                 void* d_a_xG, void* stream);
+
+/* The k-point of the complex (GPD_C128) integrate and add: phase_W holds
+ * nW complex values (interleaved re, im; host), phase_qW[q] of
+ * gpaw/lfc.py:336-343, exp(2 pi i k_q . sdisp_W) for the periodic image of
+ * each volume.  integrate then sums a * (phase dv) * A and add adds
+ * (sum_m A c) * conj(phase) (c/lfc.c:400-423, 758-781); NULL sets all
+ * ones (Gamma).                                                        */
+int gpd_lfc_set_phases(gpd_lfc* lfc, const double* phase_W);
 
 int gpd_lfc_destroy(gpd_lfc* lfc);
 

@@ -27,6 +27,9 @@
  *                                form of solid harmonics.
  *   orc_lfc_integrate/add     -- reference LFC bit-for-bit, adjointness,
  *                                dense brute force on tiny grids.
+ *   orc_lfc_integrate_z/add_z -- reference complex LFC with phase_qW
+ *                                (oracle/_ref) bit-for-bit; adjointness with
+ *                                the phases, which fixes add's conjugate.
  *   orc_precondition          -- composition of the pinned pieces; the final
  *                                axpy uses two roundings (no FMA); the
  *                                reference calls BLAS daxpy there, whose FMA
@@ -714,6 +717,97 @@ void orc_lfc_add(int nW, const double* A, const long* A_off,
     for (long x = 0; x < nx; x++) {
         lfc_ctx ctx = {A, 0, 0, 0.0, c_xM + x * nM, a_xG + x * nG};
         lfc_sweep(nW, A_off, nm_W, M_W, nB, G_B, W_B, add_seg, &ctx);
+    }
+}
+
+/* Complex (Bloch) functions (c/lfc.c:400-423, 758-781): volume W's
+ * values carry the phase phase_W[W] (interleaved re, im) of the k-point.
+ * integrate: vphase = phase * dv (each part times dv), per point
+ * av = a * vphase (C99 complex product), c[m] += av * A (each part);
+ * add: per point and volume t = sum_m A c[m] from 0, then
+ * a += t * conj(phase).                                                */
+typedef struct {
+    const double* A;
+    const double* a_G;
+    double* c_M;
+    double dv;
+    const double* cc_M;
+    double* aa_G;
+    const double* phase_W;
+} lfcz_ctx;
+
+static void integrate_seg_z(void* p, long Ga, long Gb, const active_vol* v,
+                            int ni)
+{
+    lfcz_ctx* ctx = (lfcz_ctx*)p;
+    for (int i = 0; i < ni; i++) {
+        const double* A = ctx->A + v[i].pos;
+        double* c = ctx->c_M + 2 * v[i].M;
+        int nm = v[i].nm;
+        const double vr = ctx->phase_W[2 * v[i].W] * ctx->dv;
+        const double vi = ctx->phase_W[2 * v[i].W + 1] * ctx->dv;
+        long gm = 0;
+        for (long G = Ga; G < Gb; G++) {
+            const double ar = ctx->a_G[2 * G], ai = ctx->a_G[2 * G + 1];
+            const double r1 = ar * vr, r2 = ai * vi;
+            const double i1 = ar * vi, i2 = ai * vr;
+            const double avr = r1 - r2, avi = i1 + i2;
+            for (int m = 0; m < nm; m++, gm++) {
+                c[2 * m] += avr * A[gm];
+                c[2 * m + 1] += avi * A[gm];
+            }
+        }
+    }
+}
+
+static void add_seg_z(void* p, long Ga, long Gb, const active_vol* v, int ni)
+{
+    lfcz_ctx* ctx = (lfcz_ctx*)p;
+    for (int i = 0; i < ni; i++) {
+        const double* A = ctx->A + v[i].pos;
+        const double* c = ctx->cc_M + 2 * v[i].M;
+        int nm = v[i].nm;
+        const double pr = ctx->phase_W[2 * v[i].W];
+        const double pim = -ctx->phase_W[2 * v[i].W + 1];
+        long gm = 0;
+        for (long G = Ga; G < Gb; G++) {
+            double tr = 0.0, ti = 0.0;
+            for (int m = 0; m < nm; m++, gm++) {
+                tr += A[gm] * c[2 * m];
+                ti += A[gm] * c[2 * m + 1];
+            }
+            const double r1 = tr * pr, r2 = ti * pim;
+            const double i1 = tr * pim, i2 = ti * pr;
+            ctx->aa_G[2 * G] += r1 - r2;
+            ctx->aa_G[2 * G + 1] += i1 + i2;
+        }
+    }
+}
+
+void orc_lfc_integrate_z(int nW, const double* A, const long* A_off,
+                         const int* nm_W, const int* M_W, long nB,
+                         const int* G_B, const int* W_B, double dv, long nx,
+                         long nG, const double* a_xG, long nM, double* c_xM,
+                         const double* phase_W)
+{
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < nx; x++) {
+        lfcz_ctx ctx = {A, a_xG + 2 * x * nG, c_xM + 2 * x * nM, dv, 0, 0,
+                        phase_W};
+        lfc_sweep(nW, A_off, nm_W, M_W, nB, G_B, W_B, integrate_seg_z, &ctx);
+    }
+}
+
+void orc_lfc_add_z(int nW, const double* A, const long* A_off,
+                   const int* nm_W, const int* M_W, long nB, const int* G_B,
+                   const int* W_B, long nx, long nG, const double* c_xM,
+                   long nM, double* a_xG, const double* phase_W)
+{
+#pragma omp parallel for schedule(static)
+    for (long x = 0; x < nx; x++) {
+        lfcz_ctx ctx = {A, 0, 0, 0.0, c_xM + 2 * x * nM, a_xG + 2 * x * nG,
+                        phase_W};
+        lfc_sweep(nW, A_off, nm_W, M_W, nB, G_B, W_B, add_seg_z, &ctx);
     }
 }
 

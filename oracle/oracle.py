@@ -52,6 +52,10 @@ def lib():
                                          d, l, l, _dp, l, _dp]),
             'orc_lfc_add': (None, [i, _dp, _lp, _ip, _ip, l, _ip, _ip, l, l,
                                    _dp, l, _dp]),
+            'orc_lfc_integrate_z': (None, [i, _dp, _lp, _ip, _ip, l, _ip, _ip,
+                                           d, l, l, _dp, l, _dp, _dp]),
+            'orc_lfc_add_z': (None, [i, _dp, _lp, _ip, _ip, l, _ip, _ip, l,
+                                     l, _dp, l, _dp, _dp]),
             'orc_add_local_potential': (None, [l, l, _dp, _dp, _dp]),
             'orc_projector_coefs': (None, [l, l, i, _ip, _ip, _lp, _dp, _dp,
                                            _dp]),
@@ -252,6 +256,45 @@ def lfc_add(A_Wgm, M_W, G_B, W_B, c_xM, a_xG):
     lib().orc_lfc_add(len(A_Wgm), _d(A), _l(A_off), _i(nm_W), _i(M_W),
                       len(G_B), _i(G_B), _i(W_B), nx, nG, _d(c_xM),
                       c_xM.shape[1], _d(a_xG))
+    return a_xG
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, np.complex128)
+
+
+def lfc_integrate_z(A_Wgm, M_W, G_B, W_B, dv, a_xG, Mmax, phase_W,
+                    c_xM=None):
+    """Complex LFC integrate with per-volume phases (c/lfc.c:400-423):
+    c_xM += sum_G a (phase dv) A."""
+    a_xG = _c128(a_xG)
+    nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+    nG = int(np.prod(a_xG.shape[-3:]))
+    c_xM = (np.zeros((nx, Mmax), complex) if c_xM is None
+            else _c128(c_xM).copy())
+    A, A_off, nm_W, M_W, _ = _lfc_tables(A_Wgm, M_W)
+    ph = _c128(phase_W)
+    lib().orc_lfc_integrate_z(
+        len(A_Wgm), _d(A), _l(A_off), _i(nm_W), _i(M_W), len(G_B),
+        _i(_i32(G_B)), _i(_i32(W_B)), ctypes.c_double(dv), nx, nG,
+        _d(a_xG.view(np.float64)), Mmax, _d(c_xM.view(np.float64)),
+        _d(ph.view(np.float64)))
+    return c_xM
+
+
+def lfc_add_z(A_Wgm, M_W, G_B, W_B, c_xM, a_xG, phase_W):
+    """Complex LFC add with per-volume phases (c/lfc.c:758-781): returns
+    a_xG + sum_M A c conj(phase)."""
+    a_xG = _c128(a_xG).copy()
+    nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
+    nG = int(np.prod(a_xG.shape[-3:]))
+    c_xM = _c128(c_xM).reshape(nx, -1)
+    A, A_off, nm_W, M_W, _ = _lfc_tables(A_Wgm, M_W)
+    ph = _c128(phase_W)
+    lib().orc_lfc_add_z(
+        len(A_Wgm), _d(A), _l(A_off), _i(nm_W), _i(M_W), len(G_B),
+        _i(_i32(G_B)), _i(_i32(W_B)), nx, nG, _d(c_xM.view(np.float64)),
+        c_xM.shape[1], _d(a_xG.view(np.float64)), _d(ph.view(np.float64)))
     return a_xG
 
 

@@ -211,6 +211,34 @@ def main():
     cc = rng.standard_normal((3, pt.Mmax))
     b = a.copy()
     lfc.add(cc, b, -1)
+    # complex LFC with k-point phases on a 24^3 cell smaller than the
+    # spheres' diameter (periodic images overlap themselves), two k-points
+    import types
+    gdz = grid((24, 24, 24), True, h=0.18)
+    kpts = np.array([[0.25, -0.1, 0.4], [0.5, 0.5, 0.0]])
+    ptz = LFC(gdz, [splines] * 3, kd=types.SimpleNamespace(
+        gamma=False, ibzk_qc=kpts), dtype=complex)
+    ptz.set_positions(np.random.default_rng(510).uniform(0, 1, (3, 3)))
+    lfcz = R.LFC(ptz.A_Wgm, ptz.M_W, ptz.G_B, ptz.W_B, gdz.dv,
+                 np.ascontiguousarray(ptz.phase_qW))
+    rng = np.random.default_rng(511)
+    shape = (2, 24, 24, 24)
+    az = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    ccz = (rng.standard_normal((2, ptz.Mmax)) +
+           1j * rng.standard_normal((2, ptz.Mmax)))
+    zq = []
+    for q in range(len(kpts)):
+        cz = np.zeros((2, ptz.Mmax), complex)
+        lfcz.integrate(az, cz, q)
+        bz = az.copy()
+        lfcz.add(ccz, bz, q)
+        zq.append({'integrate_sha256': digest(cz.view(np.float64)),
+                   'add_sha256': digest(bz.view(np.float64))})
+    fx['lfc_complex'] = {'N': 24, 'h_ang': 0.18, 'natoms': 3,
+                         'spos_seed': 510, 'data_seed': 511,
+                         'ibzk_qc': kpts.tolist(), 'nW': len(ptz.A_Wgm),
+                         'Mmax': int(ptz.Mmax), 'by_q': zq}
+
     fx['lfc'] = {'N': 40, 'h_ang': 0.18, 'spos_seed': 500, 'natoms': 6,
                  'data_seed': 501, 'Mmax': int(pt.Mmax),
                  'nW': len(pt.A_Wgm), 'G_B_sha256': hashlib.sha256(

@@ -162,3 +162,70 @@ def test_complex_stencil_matches_oracle_48(cuda):
     re = torch.empty_like(d.real.contiguous())
     kin.apply(d.real.contiguous(), re)
     assert torch.equal(out.real, re)
+
+
+def test_complex_lfc_on_device(cuda):
+    """Complex LFC integrate/add with the phases of two k-points (a cell
+    smaller than the spheres: 69 periodic images) equal the reference's C
+    bit for bit."""
+    import types
+    import torch
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import projector_splines
+    lz = FX['lfc_complex']
+    N = np.array([lz['N']] * 3)
+    gd = GridDescriptor(N, N * lz['h_ang'] / FX['bohr'], True)
+    pt = LFC(gd, [projector_splines(2)] * lz['natoms'],
+             kd=types.SimpleNamespace(gamma=False,
+                                      ibzk_qc=np.array(lz['ibzk_qc'])),
+             dtype=complex)
+    pt.set_positions(np.random.default_rng(lz['spos_seed']).uniform(
+        0, 1, (lz['natoms'], 3)))
+    rng = np.random.default_rng(lz['data_seed'])
+    shape = (2,) + (lz['N'],) * 3
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    cc = (rng.standard_normal((2, pt.Mmax)) +
+          1j * rng.standard_normal((2, pt.Mmax)))
+    da = torch.from_numpy(a).cuda()
+    dcc = torch.from_numpy(cc).cuda()
+    for q, ref in enumerate(lz['by_q']):
+        c = torch.zeros((2, pt.Mmax), dtype=torch.complex128, device='cuda')
+        pt.integrate_xM(da, c, q)
+        assert digest(c.cpu().numpy().view(np.float64)) == \
+            ref['integrate_sha256'], q
+        b = da.clone()
+        pt.add_xM(dcc, b, q)
+        assert digest(b.cpu().numpy().view(np.float64)) == ref['add_sha256'], q
+
+
+def test_complex_lfc_matches_oracle_48(cuda):
+    """48^3 x 4 complex bands, 8 atoms, a general k-point: integrate and
+    add bit for bit with the oracle."""
+    import types
+    import torch
+    import oracle as O
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import make_grid, projector_splines
+    gd = make_grid(48, 0.18)
+    pt = LFC(gd, [projector_splines(2)] * 8,
+             kd=types.SimpleNamespace(gamma=False,
+                                      ibzk_qc=np.array([[0.125, 0.375,
+                                                         -0.25]])),
+             dtype=complex)
+    pt.set_positions(np.random.default_rng(88).uniform(0, 1, (8, 3)))
+    rng = np.random.default_rng(89)
+    a = (rng.standard_normal((4, 48, 48, 48)) +
+         1j * rng.standard_normal((4, 48, 48, 48)))
+    cc = rng.standard_normal((4, pt.Mmax)) + 1j * rng.standard_normal(
+        (4, pt.Mmax))
+    c = torch.zeros((4, pt.Mmax), dtype=torch.complex128, device='cuda')
+    da = torch.from_numpy(a).cuda()
+    pt.integrate_xM(da, c, 0)
+    ref = O.lfc_integrate_z(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a,
+                            pt.Mmax, pt.phase_qW[0])
+    assert np.array_equal(c.cpu().numpy(), ref)
+    pt.add_xM(torch.from_numpy(cc).cuda(), da, 0)
+    ref = O.lfc_add_z(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, cc, a,
+                      pt.phase_qW[0])
+    assert np.array_equal(da.cpu().numpy(), ref)

@@ -212,3 +212,38 @@ def test_complex_stencil_matches_reference(case):
     gd, c, o, phase, a = _complex_case(cx)
     out = O.fdz(c, o, gd.pbc_c, phase, a)
     assert digest(out.view(np.float64)) == cx['fdz_sha256']
+
+
+def _complex_lfc():
+    import types
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import projector_splines
+    lz = FX['lfc_complex']
+    gd = grid((lz['N'],) * 3, True, lz['h_ang'])
+    pt = LFC(gd, [projector_splines(2)] * lz['natoms'],
+             kd=types.SimpleNamespace(gamma=False,
+                                      ibzk_qc=np.array(lz['ibzk_qc'])),
+             dtype=complex)
+    pt.set_positions(np.random.default_rng(lz['spos_seed']).uniform(
+        0, 1, (lz['natoms'], 3)))
+    rng = np.random.default_rng(lz['data_seed'])
+    shape = (2,) + (lz['N'],) * 3
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    cc = (rng.standard_normal((2, pt.Mmax)) +
+          1j * rng.standard_normal((2, pt.Mmax)))
+    return lz, gd, pt, a, cc
+
+
+def test_complex_lfc_matches_reference():
+    """Complex LFC integrate/add with the phases of two k-points on a cell
+    smaller than the spheres (69 periodic images): bit for bit with the
+    reference's C (c/lfc.c:400-423, 758-781)."""
+    lz, gd, pt, a, cc = _complex_lfc()
+    assert len(pt.A_Wgm) == lz['nW'] and pt.Mmax == lz['Mmax']
+    for q, ref in enumerate(lz['by_q']):
+        c = O.lfc_integrate_z(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a,
+                              pt.Mmax, pt.phase_qW[q])
+        assert digest(c.view(np.float64)) == ref['integrate_sha256']
+        b = O.lfc_add_z(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, cc, a,
+                        pt.phase_qW[q])
+        assert digest(b.view(np.float64)) == ref['add_sha256']

@@ -26,7 +26,7 @@ laplace = [[0],
            [-5369 / 1800, 12 / 7, -15 / 56, 10 / 189, -1 / 112, 2 / 1925,
             -1 / 16632]]
 
-_SUPPORTED_DTYPES = (float, np.float64, np.float32)
+_SUPPORTED_DTYPES = (float, np.float64, np.float32, complex, np.complex128)
 
 
 class FDOperator:
@@ -46,7 +46,7 @@ class FDOperator:
         coef_p = np.ascontiguousarray(coef_p, float)
         assert coef_p.ndim == 1 and coef_p.shape == offset_p.shape
         if dtype not in _SUPPORTED_DTYPES:
-            raise NotImplementedError('complex operators are not supported')
+            raise NotImplementedError('dtype %r is not supported' % (dtype,))
         self.dtype = dtype
         self.shape = tuple(int(x) for x in n_c)
         self.mp = mp

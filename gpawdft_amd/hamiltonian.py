@@ -50,18 +50,27 @@ def pack2(M2, tolerance=1e-10):
 
 
 class KPoint:
-    """Gamma-point, one spin: the state the eigensolvers work on."""
+    """One k-point and spin: the state the eigensolvers work on
+    (gpaw/kpoint.py).  For complex bands q indexes the LFC's phase_qW and
+    phase_cd ((3, 2) complex) is the stencil's Bloch boundary condition;
+    the defaults are the Gamma point."""
 
-    def __init__(self, psit_nG=None, s=0, weight=1.0, f_n=None):
+    def __init__(self, psit_nG=None, s=0, weight=1.0, f_n=None, q=-1,
+                 phase_cd=None):
         self.s = s
-        self.q = -1
+        self.q = q
         self.k = 0
         self.weight = weight
-        self.phase_cd = None
+        self.phase_cd = phase_cd
         self.psit_nG = psit_nG
         self.P_ani = None
         self.eps_n = None
         self.f_n = f_n
+
+
+def _real_dtype(dt):
+    """The potential and dH of complex bands stay real FP64."""
+    return torch.float64 if dt == torch.complex128 else dt
 
 
 class Hamiltonian:
@@ -111,7 +120,7 @@ class Hamiltonian:
         """Htpsit_nG += psit_nG * vt_G (gpaw/hamiltonian.py:397-424)."""
         vin = DeviceView(psit_nG)
         vout = DeviceView(Htpsit_nG)
-        vt = self._vt(s, vin.t.dtype, vin.t.device)
+        vt = self._vt(s, _real_dtype(vin.t.dtype), vin.t.device)
         nG = int(np.prod(psit_nG.shape[-3:]))
         nb = int(np.prod(psit_nG.shape[:-3], dtype=np.int64))
         check(lib().gpd_add_local_potential(vin.code, nb, nG, vin.ptr,
@@ -131,18 +140,19 @@ class Hamiltonian:
         nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
         P_xM = torch.zeros((nx, pt.Mmax), dtype=va.t.dtype,
                            device=va.t.device)
+        q = getattr(kpt, 'q', -1)
         if calculate_P_ani:
-            pt.integrate_xM(va, P_xM)
+            pt.integrate_xM(va, P_xM, q)
         else:
             P_xM = pt._pack(kpt.P_ani, tuple(a_xG.shape[:-3]), va.t)
         c_xM = self.projector_coefs(pt, P_xM, kpt.s)
-        pt.add_xM(c_xM, vb.t)
+        pt.add_xM(c_xM, vb.t, q)
         vb.finish()
 
     def projector_coefs(self, pt, P_xM, s=0, out=None):
         """c_xM = P_xM dH (block diagonal over atoms), on the device."""
-        natoms, M_a, ni_a, off, dH = self._dH_tables(pt, s, P_xM.dtype,
-                                                     P_xM.device)
+        natoms, M_a, ni_a, off, dH = self._dH_tables(
+            pt, s, _real_dtype(P_xM.dtype), P_xM.device)
         nx = P_xM.shape[0]
         c_xM = torch.empty_like(P_xM) if out is None else out
         code = DeviceView(P_xM).code

@@ -181,14 +181,17 @@ int gpd_hamiltonian_apply(gpd_operator* kin, gpd_lfc* lfc, int dtype, long nb,
 
 /* Htpsit_nG += psit_nG * vt_G for nb grids of nG points
  * (Hamiltonian.apply_local_potential, gpaw/hamiltonian.py:397-424): the
- * product is rounded, then added, as the reference's numpy expression. */
+ * product is rounded, then added, as the reference's numpy expression.
+ * For GPD_C128 psi and hpsi are complex and v is real FP64 (each part
+ * times v).                                                            */
 int gpd_add_local_potential(int dtype, long nb, long nG, const void* d_psi,
                             const void* d_v, void* d_hpsi, void* stream);
 
 /* c_xM = P_xM dH for block-diagonal dH: atom a owns columns
  * [M_a[a], M_a[a] + ni_a[a]) and the ni x ni matrix at dH[dH_off[a]]
  * (row-major, dtype).  Sums run over i in order (gpaw/hamiltonian.py:456,
- * there via numpy/BLAS).  d_M_a, d_ni_a, d_dH_off are device int arrays. */
+ * there via numpy/BLAS).  d_M_a, d_ni_a, d_dH_off are device int arrays.
+ * For GPD_C128 P and c are complex and dH is real FP64.               */
 int gpd_projector_coefs(int dtype, long nb, int Mmax, int natoms,
                         const int* d_M_a, const int* d_ni_a,
                         const int* d_dH_off, const void* d_dH,

@@ -43,6 +43,8 @@ def parse():
     p.add_argument('--no-ragged', action='store_true',
                    help='skip the ragged-batch graph measurement')
     p.add_argument('--ragged-systems', type=int, default=48)
+    p.add_argument('--no-kpoint', action='store_true',
+                   help='skip the complex (k-point) H psi timing')
     p.add_argument('--no-poisson', action='store_true',
                    help='skip the multigrid Poisson solver timing')
     p.add_argument('--copy-calibration', action='store_true',
@@ -170,6 +172,10 @@ def main():
         del psi, out, P, c, d
         torch.cuda.empty_cache()
 
+    kpoint = None
+    if not args.no_kpoint:
+        kpoint = complex_hpsi(sysm, nb, args.steps)
+
     ragged = None
     if not args.no_ragged:
         ragged = ragged_batch(args.ragged_systems, args.steps)
@@ -220,6 +226,7 @@ def main():
         'time_to_1e-8': ttt,
         'ragged_batch': ragged,
         'poisson': poisson,
+        'kpoint_c128': kpoint,
         'f32_detail': {k: f32[k] for k in ('ms_stencil', 'ms_projections',
                                            'ms_projector_add', 'precond_ms',
                                            'precond_value',
@@ -240,6 +247,56 @@ def main():
         line['hpsi_plus_precond_speedup_vs_cpu_f32'] = (
             f32['hpsi_plus_precond_value'] / cpu['hpsi_plus_precond_value'])
     print(json.dumps(line))
+
+
+def complex_hpsi(sysm, nb, steps):
+    """H psi for complex bands at a k-point (Bloch stencil, local
+    potential, phased projector integrate / P dH / add) on the same 96^3
+    system: the reference's operator sequence (gpaw/hamiltonian.py:
+    426-460) in complex128."""
+    import types
+    import numpy as np
+    import torch
+    from gpawdft_amd.hamiltonian import FDWaveFunctions, Hamiltonian, KPoint
+    from gpawdft_amd.lfc import LFC
+    gd = sysm.gd
+    k_c = np.array([0.25, 0.0, -0.375])
+    phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) * k_c[:, None])
+    pt = LFC(gd, [sysm.splines] * sysm.natoms,
+             kd=types.SimpleNamespace(gamma=False, ibzk_qc=k_c[None]),
+             dtype=complex)
+    pt.set_positions(sysm.spos_ac)
+    wfs = FDWaveFunctions(gd, pt, sysm.nn, complex)
+    ham = Hamiltonian(gd, [sysm.vt_G], sysm.dH_asp)
+    kpt = KPoint(q=0, phase_cd=phase)
+    G = int(np.prod(gd.n_c))
+    g = torch.Generator(device='cuda')
+    g.manual_seed(3)
+    psi = torch.randn((nb,) + tuple(int(n) for n in gd.n_c),
+                      dtype=torch.complex128, device='cuda', generator=g)
+    out = torch.empty_like(psi)
+    ham.apply(psi, out, wfs, kpt)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ham.apply(psi, out, wfs, kpt)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    ev[0].record()
+    wfs.kin.apply(psi, out, phase)
+    ev[1].record()
+    ev[1].synchronize()
+    ms_kin = ev[0].elapsed_time(ev[1])
+    res = {'workload': '96^3 periodic grid, %d complex128 bands, k = '
+                       '(0.25, 0, -0.375), %d atoms x 9 projectors'
+                       % (nb, sysm.natoms),
+           'ms_per_step': ms, 'value': nb * G / (ms * 1e-3),
+           'unit': 'band*gridpts/s', 'ms_stencil': ms_kin,
+           'stencil_gbs': 2 * 16 * nb * G / (ms_kin * 1e-3) / 1e9}
+    del psi, out
+    torch.cuda.empty_cache()
+    return res
 
 
 def poisson_solve(N, no_cpu):

@@ -253,6 +253,101 @@ __global__ void lfc_add_kernel(const T* __restrict__ c, T* __restrict__ a, long 
 }
 
 // ------------------------------------------------------------------ //
+// FP64 / complex exact integrate from a band-fastest copy of the covered
+// points: t[i * nx + b] = a[b * nG + cov_G[i]].  The chains are those of
+// lfc_integrate_seq_kernel (one thread per (band, function), bands
+// fastest, the reference's order), but a wave's 64 lanes now load 64
+// consecutive values instead of 64 values one grid apart.
+
+// 64 covered points x 64 bands per workgroup through LDS.
+template <typename V>
+__global__ __launch_bounds__(256) void lfc_gather_kernel(const V* __restrict__ a,
+                                                         V* __restrict__ t, long nG, long nx,
+                                                         long ncov,
+                                                         const int* __restrict__ cov_G)
+{
+    __shared__ V tile[64][65];
+    const long i0 = (long)blockIdx.x * 64;
+    const long b0 = (long)blockIdx.y * 64;
+    const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;   // ty 0..3
+    const long i = i0 + tx;
+    const long G = i < ncov ? cov_G[i] : 0;
+    for (int r = ty; r < 64; r += 4) {
+        const long b = b0 + r;
+        if (i < ncov && b < nx) tile[r][tx] = a[b * nG + G];
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const long ii = i0 + r, b = b0 + tx;
+        if (ii < ncov && b < nx) t[ii * nx + b] = tile[tx][r];
+    }
+}
+
+__global__ __launch_bounds__(64) void lfc_integrate_gathered_kernel(
+    const double* __restrict__ t, double* __restrict__ c, int Mmax, long nx, double dv,
+    const double* __restrict__ A, const int* __restrict__ fun_key, const int* __restrict__ key_M,
+    const long* __restrict__ key_off, const int* __restrict__ key_C,
+    const int* __restrict__ key_row)
+{
+    constexpr int BATCH = 16;
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * Mmax) return;
+    const long band = idx % nx;
+    const int Mi = (int)(idx / nx);
+    const int k = fun_key[Mi];
+    if (k < 0) return;
+    const int m = Mi - key_M[k];
+    double* cp = c + band * Mmax + Mi;
+    double acc = *cp;
+    const long j1 = key_off[k + 1];
+    long j = key_off[k];
+    for (; j + BATCH <= j1; j += BATCH) {
+        double x[BATCH], Av[BATCH];
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) {
+            x[q] = t[(long)key_C[j + q] * nx + band];
+            Av[q] = A[(long)key_row[j + q] + m];
+        }
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) acc = madd<double>(acc, x[q] * dv, Av[q]);
+    }
+    for (; j < j1; j++)
+        acc = madd<double>(acc, t[(long)key_C[j] * nx + band] * dv, A[(long)key_row[j] + m]);
+    *cp = acc;
+}
+
+__global__ __launch_bounds__(64) void lfc_integrate_z_gathered_kernel(
+    const double2* __restrict__ t, double2* __restrict__ c, int Mmax, long nx, double dv,
+    const double* __restrict__ A, const int* __restrict__ fun_key, const int* __restrict__ key_M,
+    const long* __restrict__ key_off, const int* __restrict__ key_C,
+    const int* __restrict__ key_row, const int* __restrict__ key_W,
+    const double2* __restrict__ phase_W)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nx * Mmax) return;
+    const long band = idx % nx;
+    const int Mi = (int)(idx / nx);
+    const int k = fun_key[Mi];
+    if (k < 0) return;
+    const int m = Mi - key_M[k];
+    double2* cp = c + band * Mmax + Mi;
+    double2 acc = *cp;
+    const long j1 = key_off[k + 1];
+    for (long j = key_off[k]; j < j1; j++) {
+        const double2 x = t[(long)key_C[j] * nx + band];
+        const double2 ph = phase_W[key_W[j]];
+        const double vr = ph.x * dv, vi = ph.y * dv;
+        const double r1 = x.x * vr, r2 = x.y * vi;
+        const double i1 = x.x * vi, i2 = x.y * vr;
+        const double avr = r1 - r2, avi = i1 + i2;
+        const double Am = A[(long)key_row[j] + m];
+        acc.x = madd<double>(acc.x, avr, Am);
+        acc.y = madd<double>(acc.y, avi, Am);
+    }
+    *cp = acc;
+}
+
+// ------------------------------------------------------------------ //
 // complex (Bloch) functions, exact order (c/lfc.c:400-423, 758-781): a
 // volume's values are multiplied by its phase for this k-point.
 
@@ -613,7 +708,7 @@ extern "C" int gpd_lfc_destroy(gpd_lfc* l)
     if (!l) return GPD_OK;
     void* ptrs[] = {l->A64, l->A32, l->ptr, l->ent, l->cov_G, l->cov_ptr, l->key_M, l->key_nm,
                     l->key_off, l->key_G, l->key_row, l->fun_key, l->ent_W, l->key_W,
-                    l->phase_W};
+                    l->phase_W, l->key_C};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     free_groups(l->gi);
@@ -671,6 +766,12 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     }
     l->nW = nW;
     std::vector<double2> ones(std::max(nW, 1), make_double2(1.0, 0.0));
+    std::vector<int> kC(kG.size());
+    {
+        std::vector<int> cov_of_G(nG, -1);
+        for (size_t i = 0; i < h.cov_G.size(); i++) cov_of_G[h.cov_G[i]] = (int)i;
+        for (size_t j = 0; j < kG.size(); j++) kC[j] = cov_of_G[kG[j]];
+    }
 
     hipError_t e = build_groups(h.key_M, h.key_nm, h.key_G, h.key_row, A, nG, l->gi);
     // sub-keys for the grouped add: the j-th visit of a point in a key's
@@ -724,6 +825,7 @@ extern "C" int gpd_lfc_create(int nW, const double* A, const int* ng_W, const in
     UP(fun_key, fkey.data(), fkey.size());
     UP(ent_W, h.ent_W.data(), h.ent_W.size());
     UP(key_W, kW.data(), kW.size());
+    UP(key_C, kC.data(), kC.size());
     UP(phase_W, ones.data(), ones.size());
 #undef UP
     if (e != hipSuccess) {
@@ -743,6 +845,41 @@ extern "C" int gpd_lfc_integrate(gpd_lfc* l, int dtype, long nx, const void* d_a
     if (nx == 0 || l->nkeys == 0) return GPD_OK;
     GPD_REQUIRE(d_a_xG && d_c_xM, GPD_ERR_ARG, "null array");
     hipStream_t st = (hipStream_t)stream;
+    if ((dtype == GPD_F64 || dtype == GPD_C128) && nx >= 16 && l->ncov > 0) {
+        // gather the covered points band-fastest into a stream-ordered
+        // temporary, then run the exact chains on it
+        const size_t esz = dtype == GPD_F64 ? sizeof(double) : sizeof(double2);
+        void* t = nullptr;
+        hipError_t e = hipMallocAsync(&t, esz * l->ncov * nx, st);
+        if (e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            return GPD_ERR_HIP;
+        }
+        dim3 gg((unsigned)((l->ncov + 63) / 64), (unsigned)((nx + 63) / 64));
+        long n = nx * l->Mmax;
+        if (dtype == GPD_F64) {
+            hipLaunchKernelGGL(lfc_gather_kernel<double>, gg, dim3(256), 0, st,
+                               (const double*)d_a_xG, (double*)t, l->nG, nx, l->ncov, l->cov_G);
+            hipLaunchKernelGGL(lfc_integrate_gathered_kernel, dim3((n + 63) / 64), dim3(64), 0, st,
+                               (const double*)t, (double*)d_c_xM, l->Mmax, nx, l->dv, l->A64,
+                               l->fun_key, l->key_M, l->key_off, l->key_C, l->key_row);
+        } else {
+            hipLaunchKernelGGL(lfc_gather_kernel<double2>, gg, dim3(256), 0, st,
+                               (const double2*)d_a_xG, (double2*)t, l->nG, nx, l->ncov,
+                               l->cov_G);
+            hipLaunchKernelGGL(lfc_integrate_z_gathered_kernel, dim3((n + 63) / 64), dim3(64), 0,
+                               st, (const double2*)t, (double2*)d_c_xM, l->Mmax, nx, l->dv,
+                               l->A64, l->fun_key, l->key_M, l->key_off, l->key_C, l->key_row,
+                               l->key_W, l->phase_W);
+        }
+        int rc = last_error_to_status(hipGetLastError());
+        e = hipFreeAsync(t, st);
+        if (rc == GPD_OK && e != hipSuccess) {
+            set_last_error(hipGetErrorString(e));
+            rc = GPD_ERR_HIP;
+        }
+        return rc;
+    }
     if (dtype == GPD_F64) {
         long n = nx * l->Mmax;
         hipLaunchKernelGGL(lfc_integrate_seq_kernel<double>, dim3((n + 63) / 64), dim3(64), 0, st,

@@ -67,6 +67,7 @@ struct gpd_lfc {
     int nW = 0;
     int* ent_W = nullptr;      // nent
     int* key_W = nullptr;      // key_off[nkeys]
+    int* key_C = nullptr;      // key_off[nkeys]: covered-point index of key_G
     double2* phase_W = nullptr;   // nW (all ones until set)
     LfcGroups gi;              // grouped integrate (FP32), from the keys
     LfcGroups ga;              // grouped add (FP32), from the sub-keys

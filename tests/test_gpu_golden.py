@@ -291,3 +291,36 @@ def test_complex_hamiltonian_at_a_k_point(cuda):
     lhs = torch.vdot(x.flatten(), hy.flatten()).item()
     rhs = torch.vdot(hx.flatten(), y.flatten()).item()
     assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize('nx', [16, 37])
+def test_gathered_exact_integrate(cuda, nx):
+    """Batches of 16 bands or more take the gather-then-integrate path in
+    FP64 and complex128: bit for bit with the oracle (ragged band count)."""
+    import types
+    import torch
+    import oracle as O
+    from gpawdft_amd.lfc import LFC
+    from gpawdft_amd.synthetic import make_grid, projector_splines
+    gd = make_grid(40, 0.18)
+    spos = np.random.default_rng(90).uniform(0, 1, (6, 3))
+    rng = np.random.default_rng(91)
+    a = rng.standard_normal((nx, 40, 40, 40))
+    pt = LFC(gd, [projector_splines(2)] * 6)
+    pt.set_positions(spos)
+    c = torch.zeros((nx, pt.Mmax), dtype=torch.float64, device='cuda')
+    pt.integrate_xM(torch.from_numpy(a).cuda(), c)
+    ref = O.lfc_integrate(pt.A_Wgm, pt.M_W, pt.G_B, pt.W_B, gd.dv, a,
+                          pt.Mmax)
+    assert np.array_equal(c.cpu().numpy(), ref)
+    ptz = LFC(gd, [projector_splines(2)] * 6,
+              kd=types.SimpleNamespace(gamma=False,
+                                       ibzk_qc=np.array([[0.2, -0.3, 0.1]])),
+              dtype=complex)
+    ptz.set_positions(spos)
+    az = a + 1j * rng.standard_normal(a.shape)
+    cz = torch.zeros((nx, ptz.Mmax), dtype=torch.complex128, device='cuda')
+    ptz.integrate_xM(torch.from_numpy(az).cuda(), cz, 0)
+    refz = O.lfc_integrate_z(ptz.A_Wgm, ptz.M_W, ptz.G_B, ptz.W_B, gd.dv, az,
+                             ptz.Mmax, ptz.phase_qW[0])
+    assert np.array_equal(cz.cpu().numpy(), refz)

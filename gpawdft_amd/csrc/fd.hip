@@ -437,6 +437,185 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
     return cf;
 }
 
+// Complex streaming axis stencil (Bloch phases): the 16 x 32 tile scheme
+// of fd_axis_kernel on double2 values, for stencils in the axis kernel's
+// order.  A value loaded across a periodic boundary is multiplied by the
+// phase of that axis and direction (axis 0 first, then 1, then 2, as the
+// reference's halo copies do); each part is summed as the real FP64
+// stencil sums it.  The ring is fetched two planes ahead (fewer live
+// registers than the real kernel's aligned ring: values are twice as wide).
+__device__ __forceinline__ double2 phase_mul(double2 v, double pr, double pim)
+{
+    const double r1 = pr * v.x, r2 = pim * v.y;
+    const double i1 = pr * v.y, i2 = pim * v.x;
+    return make_double2(r1 - r2, i1 + i2);
+}
+
+template <int R>
+__global__ __launch_bounds__(kThreads) void fd_axis_z_kernel(
+    const double2* __restrict__ in, double2* __restrict__ out, int n0, int n1, int n2,
+    int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<double, R> cf, BlochPhases ph)
+{
+    constexpr int D = 3;                      // prefetch distance (planes)
+    constexpr int W = 2 * R + 1 + D + ((2 * R + 1 + D) & 1);   // window slots (even)
+    constexpr int LW = kTX + 2 * R;
+    constexpr int LH = kTY + 2 * R;
+    constexpr int NRING = 2 * R * LW + 2 * R * kTY;
+    constexpr int RPT = (NRING + kThreads - 1) / kThreads;
+    __shared__ double2 lds[2][LH][LW];
+
+    const int tiles1 = (n1 + kTY - 1) / kTY;
+    const int tiles2 = (n2 + kTX - 1) / kTX;
+    const int zchunks = (n0 + tz - 1) / tz;
+    const long nwork = (long)nband * zchunks * tiles1 * tiles2;
+    const long w = xcd_work_index(blockIdx.x, nwork);
+    if (w < 0) return;
+    const int t2 = (int)(w % tiles2);
+    long r = w / tiles2;
+    const int t1 = (int)(r % tiles1);
+    r /= tiles1;
+    const int zc = (int)(r % zchunks);
+    const long band = r / zchunks;
+
+    const int tx = threadIdx.x % kTX;
+    const int ty = threadIdx.x / kTX;
+    const int x0 = t2 * kTX, y0 = t1 * kTY;
+    const int z0 = zc * tz, z1 = min(n0, z0 + tz);
+    const int nz = z1 - z0;
+    const long plane = (long)n1 * n2;
+    const long nG = plane * n0;
+    const double2* a = in + band * nG;
+    double2* b = out + band * nG;
+
+    // in-plane position -> flat offset (or -1) and the axis-1 / axis-2
+    // sides it wrapped across (-1: none, 0: image above, 1: below)
+    auto locate = [&](int gy, int gx, long& off, int& s1, int& s2) {
+        s1 = gy < 0 ? 1 : (gy >= n1 ? 0 : -1);
+        s2 = gx < 0 ? 1 : (gx >= n2 ? 0 : -1);
+        const int wy = wrap_index(gy, n1, pbc1);
+        const int wx = wrap_index(gx, n2, pbc2);
+        off = (wy >= 0 && wx >= 0) ? (long)wy * n2 + wx : -1;
+    };
+    const int gx = x0 + tx;
+    bool valid[kRows];
+    long own_off[kRows];
+    int own_s1[kRows], own_s2[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+        const int gy = y0 + ty + 8 * k;
+        valid[k] = gx < n2 && gy < n1;
+        locate(gy, gx, own_off[k], own_s1[k], own_s2[k]);
+    }
+    int ring_i[RPT], ring_j[RPT], ring_s1[RPT], ring_s2[RPT];
+    long ring_off[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        int s = threadIdx.x + k * kThreads;
+        int i = -1, j = -1;
+        if (s < NRING) {
+            if (s < R * LW) {
+                i = s / LW;
+                j = s % LW;
+            } else if (s < 2 * R * LW) {
+                s -= R * LW;
+                i = kTY + R + s / LW;
+                j = s % LW;
+            } else {
+                s -= 2 * R * LW;
+                i = R + s / (2 * R);
+                const int c = s % (2 * R);
+                j = c < R ? c : kTX + c;
+            }
+        }
+        ring_i[k] = i;
+        ring_j[k] = j;
+        ring_off[k] = -1;
+        ring_s1[k] = ring_s2[k] = -1;
+        if (i >= 0) locate(y0 - R + i, x0 - R + j, ring_off[k], ring_s1[k], ring_s2[k]);
+    }
+
+    auto load = [&](int z, long off, int s1, int s2) -> double2 {
+        const int wz = wrap_index(z, n0, pbc0);
+        if (wz < 0 || off < 0) return make_double2(0.0, 0.0);
+        double2 v = a[(long)wz * plane + off];
+        if (z < 0) v = phase_mul(v, ph.re[0][1], ph.im[0][1]);
+        else if (z >= n0) v = phase_mul(v, ph.re[0][0], ph.im[0][0]);
+        if (s1 >= 0) v = phase_mul(v, ph.re[1][s1], ph.im[1][s1]);
+        if (s2 >= 0) v = phase_mul(v, ph.re[2][s2], ph.im[2][s2]);
+        return v;
+    };
+
+    double2 win[kRows][W];
+#pragma unroll
+    for (int k = 0; k < kRows; k++) {
+#pragma unroll
+        for (int s = 0; s < W - 1; s++)
+            win[k][s] = load(z0 - R + s, own_off[k], own_s1[k], own_s2[k]);
+        win[k][W - 1] = make_double2(0.0, 0.0);
+    }
+    double2 ringv[2][RPT];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int k = 0; k < RPT; k++)
+            ringv[q][k] = ring_i[k] >= 0 ? load(z0 + q, ring_off[k], ring_s1[k], ring_s2[k])
+                                         : make_double2(0.0, 0.0);
+
+    for (int ib = 0; ib < nz; ib += W) {
+#pragma unroll
+        for (int ph_ = 0; ph_ < W; ph_++) {
+            const int i = ib + ph_;
+            if (i >= nz) break;
+            const int z = z0 + i;
+            const int buf = ph_ & 1;
+#pragma unroll
+            for (int k = 0; k < kRows; k++)
+                lds[buf][R + ty + 8 * k][R + tx] = win[k][(ph_ + R) % W];
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                if (ring_i[k] >= 0) lds[buf][ring_i[k]][ring_j[k]] = ringv[ph_ & 1][k];
+            lds_barrier();
+            if (i + D < nz) {
+#pragma unroll
+                for (int k = 0; k < kRows; k++)
+                    win[k][(ph_ + W - 1) % W] =
+                        load(z + W - 1 - R, own_off[k], own_s1[k], own_s2[k]);
+            }
+            if (i + 2 < nz) {
+#pragma unroll
+                for (int k = 0; k < RPT; k++)
+                    if (ring_i[k] >= 0)
+                        ringv[ph_ & 1][k] = load(z + 2, ring_off[k], ring_s1[k], ring_s2[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < kRows; k++) {
+                if (!valid[k]) continue;
+                const int li = R + ty + 8 * k, lj = R + tx;
+                const double2 x = win[k][(ph_ + R) % W];
+                double sr = 0.0, si = 0.0;
+                auto term = [&](double2 v, double c) {
+                    sr = madd<double>(sr, v.x, c);
+                    si = madd<double>(si, v.y, c);
+                };
+                term(x, cf.c0);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(lds[buf][li][lj + j], cf.c[2][j - 1]);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(lds[buf][li][lj - j], cf.c[2][R + j - 1]);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(lds[buf][li + j][lj], cf.c[1][j - 1]);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(lds[buf][li - j][lj], cf.c[1][R + j - 1]);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(win[k][(ph_ + R + j) % W], cf.c[0][j - 1]);
+#pragma unroll
+                for (int j = 1; j <= R; j++) term(win[k][(ph_ + R - j) % W], cf.c[0][R + j - 1]);
+                b[(long)z * plane + own_off[k]] = make_double2(sr, si);
+            }
+        }
+    }
+}
+
 static int pick_tz(int n0, long work_per_plane_chunk, int R, long target)
 {
     // Split axis 0 into chunks until there are about `target` workgroups
@@ -480,6 +659,32 @@ static int launch_axis_plain(const gpd_operator* op, long nb, const T* in, T* ou
     case 5: return launch_axis_R<T, 5, EPI_PLAIN, false>(op, nb, in, out, ep, st);
     case 6: return launch_axis_R<T, 6, EPI_PLAIN, false>(op, nb, in, out, ep, st);
     default: return GPD_ERR_UNSUPPORTED;
+    }
+}
+
+template <int R>
+static int launch_axis_z_R(const gpd_operator* op, long nb, const double2* in, double2* out,
+                           const BlochPhases& ph, hipStream_t st)
+{
+    const int n0 = op->n[0], n1 = op->n[1], n2 = op->n[2];
+    const int tiles1 = (n1 + kTY - 1) / kTY, tiles2 = (n2 + kTX - 1) / kTX;
+    const int tz = pick_tz(n0, nb * tiles1 * tiles2, R, op->wg_target);
+    const int zchunks = (n0 + tz - 1) / tz;
+    const long nwork = nb * zchunks * (long)tiles1 * tiles2;
+    AxisCoefs<double, R> cf = make_axis_coefs<double, R>(op);
+    hipLaunchKernelGGL((fd_axis_z_kernel<R>), dim3(xcd_grid_size(nwork)), dim3(kThreads), 0, st,
+                       in, out, n0, n1, n2, op->pbc[0], op->pbc[1], op->pbc[2], (int)nb, tz, cf,
+                       ph);
+    return last_error_to_status(hipGetLastError());
+}
+
+static int launch_axis_z(const gpd_operator* op, long nb, const double2* in, double2* out,
+                         const BlochPhases& ph, hipStream_t st)
+{
+    switch (op->range) {
+    case 1: return launch_axis_z_R<1>(op, nb, in, out, ph, st);
+    case 2: return launch_axis_z_R<2>(op, nb, in, out, ph, st);
+    default: return launch_axis_z_R<3>(op, nb, in, out, ph, st);
     }
 }
 
@@ -731,6 +936,8 @@ extern "C" int gpd_operator_apply(gpd_operator* op, int dtype, long nin, const v
                 ph.re[ax][d] = phases ? q[0] : 1.0;
                 ph.im[ax][d] = phases ? q[1] : 0.0;
             }
+        if (axis_kernel_ok<double>(op) && op->range >= 1 && op->range <= 3)
+            return launch_axis_z(op, nin, (const double2*)d_in, (double2*)d_out, ph, st);
         const long nG = (long)op->n[0] * op->n[1] * op->n[2];
         const long blocks = std::min<long>((nG + 255) / 256, 4096);
         for (long b0 = 0; b0 < nin; b0 += 65535) {

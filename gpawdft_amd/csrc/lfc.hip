@@ -332,18 +332,31 @@ __global__ __launch_bounds__(64) void lfc_integrate_z_gathered_kernel(
     const int m = Mi - key_M[k];
     double2* cp = c + band * Mmax + Mi;
     double2 acc = *cp;
-    const long j1 = key_off[k + 1];
-    for (long j = key_off[k]; j < j1; j++) {
-        const double2 x = t[(long)key_C[j] * nx + band];
-        const double2 ph = phase_W[key_W[j]];
+    auto step = [&](double2 x, double2 ph, double Am) {
         const double vr = ph.x * dv, vi = ph.y * dv;
         const double r1 = x.x * vr, r2 = x.y * vi;
         const double i1 = x.x * vi, i2 = x.y * vr;
         const double avr = r1 - r2, avi = i1 + i2;
-        const double Am = A[(long)key_row[j] + m];
         acc.x = madd<double>(acc.x, avr, Am);
         acc.y = madd<double>(acc.y, avi, Am);
+    };
+    constexpr int BATCH = 8;
+    const long j1 = key_off[k + 1];
+    long j = key_off[k];
+    for (; j + BATCH <= j1; j += BATCH) {
+        double2 x[BATCH], ph[BATCH];
+        double Av[BATCH];
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) {
+            x[q] = t[(long)key_C[j + q] * nx + band];
+            ph[q] = phase_W[key_W[j + q]];
+            Av[q] = A[(long)key_row[j + q] + m];
+        }
+#pragma unroll
+        for (int q = 0; q < BATCH; q++) step(x[q], ph[q], Av[q]);
     }
+    for (; j < j1; j++)
+        step(t[(long)key_C[j] * nx + band], phase_W[key_W[j]], A[(long)key_row[j] + m]);
     *cp = acc;
 }
 

@@ -63,6 +63,21 @@ inline long xcd_grid_size(long nwork)
 
 void set_last_error(const char* msg);
 
+// Bloch phases of complex arrays, [axis][0] for the periodic image above
+// the cell and [axis][1] for the image below, and the C99 complex product
+// phase * v of the reference's build (two roundings per part).
+struct BlochPhases {
+    double re[3][2];
+    double im[3][2];
+};
+
+__device__ __forceinline__ double2 phase_mul(double2 v, double pr, double pim)
+{
+    const double r1 = pr * v.x, r2 = pim * v.y;
+    const double i1 = pr * v.y, i2 = pim * v.x;
+    return make_double2(r1 - r2, i1 + i2);
+}
+
 // MFMA operand and accumulator vectors
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));

@@ -70,11 +70,6 @@ __global__ void fd_generic_kernel(const T* __restrict__ in, T* __restrict__ out,
 // is then summed from 0 in coefficient order with two roundings per term,
 // as the reference's phased halo and bmgs_fd for double complex do.
 
-struct BlochPhases {
-    double re[3][2];
-    double im[3][2];
-};
-
 __global__ void fd_complex_kernel(const double2* __restrict__ in, double2* __restrict__ out,
                                   int n0, int n1, int n2, int pbc0, int pbc1, int pbc2,
                                   int ncoefs, const int* __restrict__ off,
@@ -444,13 +439,6 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
 // reference's halo copies do); each part is summed as the real FP64
 // stencil sums it.  The ring is fetched two planes ahead (fewer live
 // registers than the real kernel's aligned ring: values are twice as wide).
-__device__ __forceinline__ double2 phase_mul(double2 v, double pr, double pim)
-{
-    const double r1 = pr * v.x, r2 = pim * v.y;
-    const double i1 = pr * v.y, i2 = pim * v.x;
-    return make_double2(r1 - r2, i1 + i2);
-}
-
 template <int R>
 __global__ __launch_bounds__(kThreads) void fd_axis_z_kernel(
     const double2* __restrict__ in, double2* __restrict__ out, int n0, int n1, int n2,

@@ -27,15 +27,25 @@ template <> struct Weights<8> {
     static constexpr double w[4] = {0.59814453125, -0.11962890625, 0.02392578125, -0.00244140625};
 };
 
+// w * v for a real weight: the weight in the value's precision for real
+// types; for complex values each part times the FP64 weight (a real times
+// a complex value, as the reference's C computes it).  (double2)w would
+// not do: HIP's vector constructor from one scalar leaves y = 0.
+template <typename T> __device__ __forceinline__ T wmul(double w, T v) { return (T)w * v; }
+template <> __device__ __forceinline__ double2 wmul<double2>(double w, double2 v)
+{
+    return make_double2(w * v.x, w * v.y);
+}
+
 // Restriction of one output from a line accessor f(t) = value at offset t
 // from the centre tap: 0.5 * (f0 + w1 (f1 + f-1) + w3 (f3 + f-3) + ...).
 template <int K, typename T, typename F>
 __device__ __forceinline__ T restrict_point(F f)
 {
-    T s = f(0) + (T)Weights<K>::w[0] * (f(1) + f(-1));
+    T s = f(0) + wmul<T>(Weights<K>::w[0], f(1) + f(-1));
 #pragma unroll
-    for (int j = 1; j < K / 2; j++) s = s + (T)Weights<K>::w[j] * (f(2 * j + 1) + f(-2 * j - 1));
-    return (T)0.5 * s;
+    for (int j = 1; j < K / 2; j++) s = s + wmul<T>(Weights<K>::w[j], f(2 * j + 1) + f(-2 * j - 1));
+    return wmul<T>(0.5, s);
 }
 
 // Interpolation of the odd point between taps 0 and 1:
@@ -43,9 +53,9 @@ __device__ __forceinline__ T restrict_point(F f)
 template <int K, typename T, typename F>
 __device__ __forceinline__ T interp_point(F f)
 {
-    T s = (T)Weights<K>::w[0] * (f(0) + f(1));
+    T s = wmul<T>(Weights<K>::w[0], f(0) + f(1));
 #pragma unroll
-    for (int j = 1; j < K / 2; j++) s = s + (T)Weights<K>::w[j] * (f(-j) + f(j + 1));
+    for (int j = 1; j < K / 2; j++) s = s + wmul<T>(Weights<K>::w[j], f(-j) + f(j + 1));
     return s;
 }
 
@@ -64,11 +74,12 @@ struct TransformDims {
 // ------------------------------------------------------------------ //
 // restriction: output tile C0 x C1 x C2; input span per axis 2C + 2K - 3
 
-template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG_IN>
+template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG_IN, bool PH>
 __global__ __launch_bounds__(256) void restrict_kernel(const T* __restrict__ in,
                                                        T* __restrict__ out,
                                                        T* __restrict__ out2, T alpha,
-                                                       TransformDims d, int nband)
+                                                       TransformDims d, int nband,
+                                                       BlochPhases ph)
 {
     constexpr int H = K - 1;                 // taps on each side
     constexpr int S0 = 2 * C0 + 2 * H - 1, S1 = 2 * C1 + 2 * H - 1;
@@ -102,6 +113,16 @@ __global__ __launch_bounds__(256) void restrict_kernel(const T* __restrict__ in,
         int j2 = wrap_index(i2, a2.n_in, a2.pbc);
         if (j0 < 0 || j1 < 0 || j2 < 0) return T(0);
         T v = src[((long)j0 * a1.n_in + j1) * a2.n_in + j2];
+        if constexpr (PH) {
+            // periodic image: the phase of each axis crossed, axis 0 first
+            const int ii[3] = {i0, i1, i2};
+            const int nn[3] = {a0.n_in, a1.n_in, a2.n_in};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (ii[c] < 0) v = phase_mul(v, ph.re[c][1], ph.im[c][1]);
+                else if (ii[c] >= nn[c]) v = phase_mul(v, ph.re[c][0], ph.im[c][0]);
+            }
+        }
         return NEG_IN ? -v : v;
     };
 
@@ -140,10 +161,11 @@ __global__ __launch_bounds__(256) void restrict_kernel(const T* __restrict__ in,
 // interpolation: output tile F0 x F1 x F2 (even), in unskipped positions
 // u = o + skip0; input span per axis F/2 + K - 1
 
-template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG_IN>
+template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG_IN, bool PH>
 __global__ __launch_bounds__(256) void interpolate_kernel(const T* __restrict__ in,
                                                           T* __restrict__ out,
-                                                          TransformDims d, int nband)
+                                                          TransformDims d, int nband,
+                                                          BlochPhases ph)
 {
     constexpr int H = K / 2 - 1;             // taps left of tap 0
     constexpr int S0 = F0 / 2 + K - 1, S1 = F1 / 2 + K - 1;
@@ -180,6 +202,16 @@ __global__ __launch_bounds__(256) void interpolate_kernel(const T* __restrict__ 
         int j2 = wrap_index(i2, a2.n_in, a2.pbc);
         if (j0 < 0 || j1 < 0 || j2 < 0) return T(0);
         T v = src[((long)j0 * a1.n_in + j1) * a2.n_in + j2];
+        if constexpr (PH) {
+            // periodic image: the phase of each axis crossed, axis 0 first
+            const int ii[3] = {i0, i1, i2};
+            const int nn[3] = {a0.n_in, a1.n_in, a2.n_in};
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                if (ii[c] < 0) v = phase_mul(v, ph.re[c][1], ph.im[c][1]);
+                else if (ii[c] >= nn[c]) v = phase_mul(v, ph.re[c][0], ph.im[c][0]);
+            }
+        }
         return NEG_IN ? -v : v;
     };
 
@@ -248,21 +280,23 @@ static TransformDims make_dims(const gpd_transformer* t)
     return d;
 }
 
-template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG>
+template <typename T, int K, int C0, int C1, int C2, int EPI, bool NEG, bool PH = false>
 static int launch_restrict(const gpd_transformer* t, long nb, const T* in, T* out, T* out2,
-                           T alpha, hipStream_t st)
+                           T alpha, hipStream_t st, const BlochPhases& ph = BlochPhases{})
 {
     TransformDims d = make_dims(t);
     long tiles = (long)((t->n_out[0] + C0 - 1) / C0) * ((t->n_out[1] + C1 - 1) / C1) *
                  ((t->n_out[2] + C2 - 1) / C2);
     long nwork = tiles * nb;
-    hipLaunchKernelGGL((restrict_kernel<T, K, C0, C1, C2, EPI, NEG>), dim3(xcd_grid_size(nwork)),
-                       dim3(256), 0, st, in, out, out2, alpha, d, (int)nb);
+    hipLaunchKernelGGL((restrict_kernel<T, K, C0, C1, C2, EPI, NEG, PH>),
+                       dim3(xcd_grid_size(nwork)), dim3(256), 0, st, in, out, out2, alpha, d,
+                       (int)nb, ph);
     return last_error_to_status(hipGetLastError());
 }
 
-template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG>
-static int launch_interp(const gpd_transformer* t, long nb, const T* in, T* out, hipStream_t st)
+template <typename T, int K, int F0, int F1, int F2, int EPI, bool NEG, bool PH = false>
+static int launch_interp(const gpd_transformer* t, long nb, const T* in, T* out, hipStream_t st,
+                         const BlochPhases& ph = BlochPhases{})
 {
     TransformDims d = make_dims(t);
     long tiles = 1;
@@ -271,8 +305,8 @@ static int launch_interp(const gpd_transformer* t, long nb, const T* in, T* out,
         tiles *= (d.ax[c].n_out + d.ax[c].skip0 + F - 1) / F;
     }
     long nwork = tiles * nb;
-    hipLaunchKernelGGL((interpolate_kernel<T, K, F0, F1, F2, EPI, NEG>), dim3(xcd_grid_size(nwork)),
-                       dim3(256), 0, st, in, out, d, (int)nb);
+    hipLaunchKernelGGL((interpolate_kernel<T, K, F0, F1, F2, EPI, NEG, PH>),
+                       dim3(xcd_grid_size(nwork)), dim3(256), 0, st, in, out, d, (int)nb, ph);
     return last_error_to_status(hipGetLastError());
 }
 
@@ -323,6 +357,32 @@ int transformer_apply_typed(const gpd_transformer* t, int epi, bool neg_in, long
     return GPD_ERR_ARG;
 }
 
+// complex arrays with Bloch phases: Transformer.apply only (no epilogues);
+// tiles as the FP64 ones, the LDS of an 8-wide restrict tile at k >= 6
+// halved along axis 2 to stay within the workgroup's LDS
+static int transformer_apply_z(const gpd_transformer* t, long nb, const double2* in,
+                               double2* out, const BlochPhases& ph, hipStream_t st)
+{
+    double2* none = nullptr;
+    const double2 zero = make_double2(0.0, 0.0);
+    if (!t->interpolate) {
+        switch (t->k) {
+        case 2: return launch_restrict<double2, 2, 4, 8, 16, 0, false, true>(t, nb, in, out, none, zero, st, ph);
+        case 4: return launch_restrict<double2, 4, 4, 8, 16, 0, false, true>(t, nb, in, out, none, zero, st, ph);
+        case 6: return launch_restrict<double2, 6, 2, 4, 16, 0, false, true>(t, nb, in, out, none, zero, st, ph);
+        case 8: return launch_restrict<double2, 8, 2, 4, 8, 0, false, true>(t, nb, in, out, none, zero, st, ph);
+        default: return GPD_ERR_ARG;
+        }
+    }
+    switch (t->k) {
+    case 2: return launch_interp<double2, 2, 8, 8, 32, 0, false, true>(t, nb, in, out, st, ph);
+    case 4: return launch_interp<double2, 4, 8, 8, 16, 0, false, true>(t, nb, in, out, st, ph);
+    case 6: return launch_interp<double2, 6, 4, 8, 16, 0, false, true>(t, nb, in, out, st, ph);
+    case 8: return launch_interp<double2, 8, 4, 8, 16, 0, false, true>(t, nb, in, out, st, ph);
+    default: return GPD_ERR_ARG;
+    }
+}
+
 template int transformer_apply_typed<double>(const gpd_transformer*, int, bool, long,
                                              const double*, double*, double*, double,
                                              hipStream_t);
@@ -342,7 +402,7 @@ extern "C" int gpd_transformer_create(const long size_in[3], const long size_out
                 GPD_ERR_ARG, "null argument");
     *out = nullptr;
     GPD_REQUIRE(k == 2 || k == 4 || k == 6 || k == 8, GPD_ERR_ARG, "k must be 2, 4, 6 or 8");
-    GPD_REQUIRE(real == 1, GPD_ERR_UNSUPPORTED, "complex transformers are not supported");
+    (void)real;   // the object serves real and complex arrays alike
     gpd_transformer* t = new (std::nothrow) gpd_transformer();
     GPD_REQUIRE(t, GPD_ERR_NOMEM, "out of host memory");
     t->k = k;
@@ -380,11 +440,26 @@ extern "C" int gpd_transformer_apply(gpd_transformer* t, int dtype, long nin, co
                                      void* d_out, const void* phases, void* stream)
 {
     GPD_REQUIRE(t, GPD_ERR_ARG, "null transformer");
-    GPD_REQUIRE(!phases, GPD_ERR_UNSUPPORTED, "Bloch phases are not supported");
+    GPD_REQUIRE(!phases || dtype == GPD_C128, GPD_ERR_ARG,
+                "phases apply to complex arrays only");
     GPD_REQUIRE(nin >= 0, GPD_ERR_ARG, "negative batch");
     if (nin == 0) return GPD_OK;
     GPD_REQUIRE(d_in && d_out && d_in != d_out, GPD_ERR_ARG, "in and out must be distinct");
     hipStream_t st = (hipStream_t)stream;
+    if (dtype == GPD_C128) {
+        // an image is crossed at most once
+        for (int c = 0; c < 3; c++)
+            GPD_REQUIRE(t->pad[c][0] <= t->n_in[c] && t->pad[c][1] <= t->n_in[c],
+                        GPD_ERR_ARG, "padding reaches beyond one periodic image");
+        BlochPhases ph;
+        for (int c = 0; c < 3; c++)
+            for (int d = 0; d < 2; d++) {
+                const double* q = (const double*)phases + 4 * c + 2 * d;
+                ph.re[c][d] = phases ? q[0] : 1.0;
+                ph.im[c][d] = phases ? q[1] : 0.0;
+            }
+        return transformer_apply_z(t, nin, (const double2*)d_in, (double2*)d_out, ph, st);
+    }
     if (dtype == GPD_F64)
         return transformer_apply_typed<double>(t, TEPI_PLAIN, false, nin, (const double*)d_in,
                                                (double*)d_out, nullptr, 0.0, st);

@@ -82,8 +82,9 @@ class _Transformer:
         return self._handle
 
     def apply(self, input, output=None, phases=None):
-        if phases is not None and np.iscomplexobj(np.asarray(phases)):
-            raise NotImplementedError('Bloch phases are not supported')
+        """Restrict or interpolate a batch of grids.  For complex128 grids
+        phases ((3, 2) complex, [c][0] for the periodic image above, [c][1]
+        below) gives the Bloch boundary condition; None means Gamma."""
         if tuple(input.shape[-3:]) != self.ngpin:
             raise ValueError('input grid %s does not match %s'
                              % (tuple(input.shape[-3:]), self.ngpin))
@@ -102,10 +103,14 @@ class _Transformer:
         if vin.code != vout.code:
             raise TypeError('in/out dtypes differ')
         nin = int(np.prod(input.shape[:-3], dtype=np.int64))
-        check(lib().gpd_transformer_apply(self._handle, vin.code, nin,
-                                          vin.ptr, vout.ptr, None,
-                                          stream_ptr()),
-              'Transformer.apply')
+        ph = None
+        if vin.code == _lib.GPD_C128 and phases is not None:
+            ph = np.ascontiguousarray(np.asarray(phases, np.complex128)
+                                      .reshape(3, 2)).view(np.float64)
+        check(lib().gpd_transformer_apply(
+            self._handle, vin.code, nin, vin.ptr, vout.ptr,
+            None if ph is None else ph.ctypes.data, stream_ptr()),
+            'Transformer.apply')
         vout.finish()
         return output
 

@@ -99,7 +99,10 @@ int gpd_transformer_create(const long size_in[3], const long size_out[3],
                            const long neighbors[6], int real, int interpolate,
                            gpd_transformer** out);
 
-/* Transformer.apply(in, out, phases) (c/transformers.c:127-185). */
+/* Transformer.apply(in, out, phases) (c/transformers.c:127-185).  For
+ * GPD_F64 / GPD_F32 phases must be NULL; for GPD_C128 it is the (3, 2)
+ * complex phase_cd as in gpd_operator_apply (12 host doubles, NULL for
+ * Gamma), bit-compatible with the reference's complex Transformer.    */
 int gpd_transformer_apply(gpd_transformer* t, int dtype, long nin,
                           const void* d_in, void* d_out, const void* phases,
                           void* stream);

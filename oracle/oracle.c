@@ -16,6 +16,8 @@
  * Parity status (how each function is pinned):
  *   orc_halo_fill, orc_fd     -- reference Operator (oracle/_ref) bit-for-bit,
  *                                FD symbol closed form, polynomial exactness.
+ *   orc_transform_z           -- reference complex Transformer with phases
+ *                                (oracle/_ref) bit-for-bit.
  *   orc_halo_fill_z, orc_fdz  -- reference complex Operator with phases
  *                                (oracle/_ref) bit-for-bit; Bloch plane waves
  *                                are eigenvectors with the FD symbol at k+G.
@@ -458,6 +460,63 @@ void orc_transform(int k, int interpolate, const long n[3], const long lo[3],
         free(t);
         free(w);
         free(buf);
+    }
+}
+
+/* Complex Transformer apply with Bloch phases (c/transformers.c:54-123
+ * with bmgs_restrictz / bmgs_interpolatez): the phased halo of
+ * orc_halo_fill_z, then the same three passes; a real weight times a
+ * complex value and complex sums are per part, so the passes run on the
+ * real and the imaginary parts separately.  in/out interleaved.         */
+void orc_transform_z(int k, int interpolate, const long n[3], const long lo[3],
+                     const long hi[3], const int pbc[3], const int* skip,
+                     const long nout[3], const double* phase, long nb,
+                     const double* in, double* out)
+{
+    long m[3] = {n[0] + lo[0] + hi[0], n[1] + lo[1] + hi[1],
+                 n[2] + lo[2] + hi[2]};
+    long ng = n[0] * n[1] * n[2];
+    long ngo = nout[0] * nout[1] * nout[2];
+    long ng2 = m[0] * m[1] * m[2];
+#pragma omp parallel
+    {
+        double* zbuf = (double*)malloc(sizeof(double) * 2 * ng2);
+        double* buf = (double*)malloc(sizeof(double) * ng2);
+        double* w = (double*)malloc(sizeof(double) * ng2 * 16);
+        double* t = (double*)malloc(sizeof(double) * ng2 * 16);
+        double* o = (double*)malloc(sizeof(double) * ngo);
+#pragma omp for schedule(static)
+        for (long x = 0; x < nb; x++) {
+            orc_halo_fill_z(in + 2 * x * ng, n, lo, hi, pbc, 1, phase, zbuf);
+            for (int part = 0; part < 2; part++) {
+                for (long g = 0; g < ng2; g++) buf[g] = zbuf[2 * g + part];
+                if (!interpolate) {
+                    long e = 2 * k - 3;
+                    long o2 = (m[2] - e) / 2, o1 = (m[1] - e) / 2,
+                         o0 = (m[0] - e) / 2;
+                    rst_pass(k, buf, m[2], o2, m[0] * m[1], w);
+                    rst_pass(k, w, m[1], o1, m[0] * o2, t);
+                    rst_pass(k, t, m[0], o0, o1 * o2, o);
+                } else {
+                    long e = k - 1;
+                    const int* s2 = skip + 4;
+                    const int* s1 = skip + 2;
+                    const int* s0 = skip;
+                    long f2 = (m[2] - e) * 2 - s2[0] + s2[1];
+                    long f1 = (m[1] - e) * 2 - s1[0] + s1[1];
+                    ip_pass(k, buf, m[2] - e + s2[1], m[0] * m[1], s2, t);
+                    ip_pass(k, t, m[1] - e + s1[1], m[0] * f2, s1, w);
+                    ip_pass(k, w, m[0] - e + s0[1], f1 * f2, s0, o);
+                }
+                double* dst = out + 2 * x * ngo;
+                for (long g = 0; g < ngo; g++) dst[2 * g + part] = o[g];
+            }
+        }
+        free(o);
+        free(t);
+        free(w);
+        free(buf);
+        free(zbuf);
     }
 }
 

@@ -44,6 +44,8 @@ def lib():
                                  d]),
             'orc_transform': (None, [i, i, _lp, _lp, _lp, _ip, _ip, _lp, l,
                                      _dp, _dp]),
+            'orc_transform_z': (None, [i, i, _lp, _lp, _lp, _ip, _ip, _lp,
+                                       _dp, l, _dp, _dp]),
             'orc_spline_build': (None, [d, i, _dp, _dp]),
             'orc_spline_value': (d, [_dp, i, d, d]),
             'orc_spline_to_grid': (l, [_dp, i, d, i, _lp, _lp, _dp, _dp, _lp,
@@ -164,6 +166,27 @@ def transform(k, interpolate, pad_cd, skip_cd, nout_c, pbc_c, in_xg):
     pbc = _i32(np.asarray(pbc_c, bool).astype(int))
     lib().orc_transform(k, int(interpolate), _l(n), _l(lo), _l(hi), _i(pbc),
                         _i(skip), _l(nout), nb, _d(in_xg), _d(out))
+    return out
+
+
+def transform_z(k, interpolate, pad_cd, skip_cd, nout_c, pbc_c, phase_cd,
+                in_xg):
+    """Complex Transformer apply with Bloch phases (c/transformers.c:54-123
+    with bmgs_restrictz / bmgs_interpolatez)."""
+    in_xg = np.ascontiguousarray(in_xg, np.complex128)
+    n = _i64(in_xg.shape[-3:])
+    pad = np.asarray(pad_cd, int)
+    skip = _i32(np.asarray(skip_cd, int).reshape(-1))
+    nout = _i64(nout_c)
+    nb = int(np.prod(in_xg.shape[:-3], dtype=np.int64))
+    out = np.empty(in_xg.shape[:-3] + tuple(int(x) for x in nout), complex)
+    pbc = _i32(np.asarray(pbc_c, bool).astype(int))
+    ph = np.ascontiguousarray(np.asarray(phase_cd, np.complex128)
+                              .reshape(3, 2)).view(np.float64)
+    lib().orc_transform_z(k, int(interpolate), _l(n), _l(_i64(pad[:, 0])),
+                          _l(_i64(pad[:, 1])), _i(pbc), _i(skip), _l(nout),
+                          _d(ph), nb, _d(in_xg.view(np.float64)),
+                          _d(out.view(np.float64)))
     return out
 
 

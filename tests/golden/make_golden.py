@@ -62,11 +62,11 @@ def ref_operator(R, gd, coefs, offs, real=1):
                       None, int(cfd))
 
 
-def ref_transformer(R, gdin, gdout, nn):
+def ref_transformer(R, gdin, gdout, nn, real=1):
     pad, npad, skip, interp = transformer_paddings(gdin, gdout, nn)
     i64 = lambda x: np.ascontiguousarray(x, np.int64)
     t = R.Transformer(i64(gdin.n_c), i64(gdout.n_c), 2 * nn, i64(pad),
-                      i64(npad), i64(skip), i64(gdin.neighbor_cd), 1, None,
+                      i64(npad), i64(skip), i64(gdin.neighbor_cd), real, None,
                       int(interp))
     return t
 
@@ -168,6 +168,32 @@ def main():
                        'seed': seed,
                        'fdz_sha256': digest(out.view(np.float64))})
     fx['complex_fd'] = zcases
+
+    # complex transformers with phases: restrict and interpolate, nn = 1
+    # and 3, periodic and mixed boundaries
+    tz = []
+    for N, pbc, nn, k_c, seed in (((24, 24, 24), (1, 1, 1), 1,
+                                   (0.25, -0.1, 0.4), 420),
+                                  ((24, 20, 28), (1, 1, 1), 3,
+                                   (0.5, 0.125, -0.3), 421),
+                                  ((24, 24, 16), (1, 0, 1), 2,
+                                   (0.2, 0.0, -0.45), 422)):
+        gd = grid(N, pbc)
+        gdc = gd.coarsen()
+        phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                       np.array(k_c)[:, None])
+        rng = np.random.default_rng(seed)
+        shape = (2,) + tuple(int(x) for x in gd.n_c)
+        a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        r = np.empty((2,) + tuple(int(x) for x in gdc.n_c), complex)
+        ref_transformer(R, gd, gdc, nn, real=0).apply(a, r, phase)
+        i = np.empty_like(a)
+        ref_transformer(R, gdc, gd, nn, real=0).apply(r, i, phase)
+        tz.append({'N': list(N), 'pbc': list(pbc), 'nn': nn,
+                   'k_c': list(k_c), 'seed': seed,
+                   'restrict_sha256': digest(r.view(np.float64)),
+                   'interpolate_sha256': digest(i.view(np.float64))})
+    fx['complex_transform'] = tz
 
     # splines and spline_to_grid
     rc, rmax = 1.2 / Bohr, 2.5 / Bohr

@@ -324,3 +324,56 @@ def test_gathered_exact_integrate(cuda, nx):
     refz = O.lfc_integrate_z(ptz.A_Wgm, ptz.M_W, ptz.G_B, ptz.W_B, gd.dv, az,
                              ptz.Mmax, ptz.phase_qW[0])
     assert np.array_equal(cz.cpu().numpy(), refz)
+
+
+@pytest.mark.parametrize('case', range(3))
+def test_complex_transformers_on_device(cuda, case):
+    """Complex restrict and interpolate with Bloch phases equal the
+    reference's C bit for bit (nn = 1, 2, 3; periodic and mixed)."""
+    import torch
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.transformers import Transformer
+    cx = FX['complex_transform'][case]
+    N = np.array(cx['N'])
+    gd = GridDescriptor(N, N * 0.2 / FX['bohr'], tuple(cx['pbc']))
+    gdc = gd.coarsen()
+    phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                   np.array(cx['k_c'])[:, None])
+    rng = np.random.default_rng(cx['seed'])
+    shape = (2,) + tuple(int(x) for x in gd.n_c)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    r = Transformer(gd, gdc, cx['nn'], complex).apply(
+        torch.from_numpy(a).cuda(), phases=phase)
+    assert digest(r.cpu().numpy().view(np.float64)) == cx['restrict_sha256']
+    i = Transformer(gdc, gd, cx['nn'], complex).apply(r, phases=phase)
+    assert digest(i.cpu().numpy().view(np.float64)) == \
+        cx['interpolate_sha256']
+
+
+@pytest.mark.parametrize('nn', [1, 4])
+def test_complex_transformers_match_oracle_48(cuda, nn):
+    """48^3 <-> 24^3, 4 complex bands, k = (0.125, 0.375, -0.25): bit for
+    bit with the oracle (nn = 1: the preconditioner's; nn = 4: the widest)."""
+    import torch
+    import oracle as O
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.transformers import Transformer, transformer_paddings
+    N = np.array([48, 48, 48])
+    gd = GridDescriptor(N, N * 0.18 / FX['bohr'], True)
+    gdc = gd.coarsen()
+    phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                   np.array([0.125, 0.375, -0.25])[:, None])
+    rng = np.random.default_rng(78)
+    a = (rng.standard_normal((4, 48, 48, 48)) +
+         1j * rng.standard_normal((4, 48, 48, 48)))
+    r = Transformer(gd, gdc, nn, complex).apply(torch.from_numpy(a).cuda(),
+                                                phases=phase)
+    pad, npad, skip, interp = transformer_paddings(gd, gdc, nn)
+    ref = O.transform_z(2 * nn, interp, pad, skip, gdc.n_c, gd.pbc_c, phase,
+                        a)
+    assert np.array_equal(r.cpu().numpy(), ref)
+    i = Transformer(gdc, gd, nn, complex).apply(r, phases=phase)
+    pad, npad, skip, interp = transformer_paddings(gdc, gd, nn)
+    ref = O.transform_z(2 * nn, interp, pad, skip, gd.n_c, gdc.pbc_c, phase,
+                        ref)
+    assert np.array_equal(i.cpu().numpy(), ref)

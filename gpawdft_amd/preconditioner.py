@@ -5,7 +5,9 @@ block=1)`, `__call__(residuals, kpt, ekin=None) -> d0`,
 `calculate_kinetic_energy(psit_xG, kpt) -> None`.  Same V-cycle (nn=1
 transformers, 7-point Laplacians on the coarse grids, step =
 0.66666666 / diag(kin0)), executed as eight fused gfx950 launches that
-keep the reference's per-element arithmetic.
+keep the reference's per-element arithmetic.  Complex bands (k-points)
+run the reference's call sequence on the phased stencils and
+transformers, with kpt.phase_cd.
 """
 import numpy as np
 
@@ -70,8 +72,66 @@ class Preconditioner:
                                      device=residuals.device)
         vin = DeviceView(residuals)
         vout = DeviceView(out, copy_in=False)
+        if vin.code == _lib.GPD_C128:
+            phases = None if kpt is None else getattr(kpt, 'phase_cd', None)
+            self._complex_vcycle(vin.t, vout.t, phases)
+            vout.finish()
+            return out
         check(lib().gpd_precond_apply(self._handle, vin.code, nb, vin.ptr,
                                       vout.ptr, stream_ptr()),
               'Preconditioner')
         vout.finish()
         return out
+
+    def _complex_vcycle(self, R, d_out, phases):
+        """Complex bands at a k-point: the reference's sequence of calls
+        (gpaw/preconditioner.py:34-60) on this library's phased stencils
+        and transformers.  Each numpy update y -= c * x of the reference is
+        y + (-c) * x on the interleaved FP64 parts (the same roundings:
+        (-c) * x = -(c * x) and y + (-p) = y - p exactly); the final
+        axpy(-step, q0, d0) and d0 *= -1.0 likewise."""
+        import torch
+        step = self.step
+        nb = R.shape[0]
+
+        def axpy_z(c, x, y):
+            """y += c * x for real c on complex grids (per part)."""
+            n2 = 2 * int(np.prod(x.shape[1:]))
+            coef = torch.full((nb,), float(c), dtype=torch.float64,
+                              device=x.device)
+            check(lib().gpd_axpy_batched(_lib.GPD_F64, nb, n2,
+                                         coef.data_ptr(), x.data_ptr(),
+                                         y.data_ptr(), stream_ptr()),
+                  'axpy')
+
+        def kin(op, x):
+            y = torch.empty_like(x)
+            op.apply(x, y, phases)
+            return y
+
+        neg_R = torch.zeros_like(R)
+        axpy_z(-1.0, R, neg_R)
+        r1 = self.restrictor0(neg_R, phases=phases)
+        d1 = torch.zeros_like(r1)
+        axpy_z(4 * step, r1, d1)
+        q1 = kin(self.kin1, d1)
+        axpy_z(-1.0, r1, q1)
+        r2 = self.restrictor1(q1, phases=phases)
+        d2 = torch.zeros_like(r2)
+        axpy_z(16 * step, r2, d2)
+        q2 = kin(self.kin2, d2)
+        axpy_z(-1.0, r2, q2)
+        axpy_z(-(16 * step), q2, d2)
+        q1 = self.interpolator2(d2, phases=phases)
+        axpy_z(-1.0, q1, d1)
+        q1 = kin(self.kin1, d1)
+        axpy_z(-1.0, r1, q1)
+        axpy_z(-(4 * step), q1, d1)
+        neg_d1 = torch.zeros_like(d1)
+        axpy_z(-1.0, d1, neg_d1)
+        d0 = self.interpolator1(neg_d1, phases=phases)
+        q0 = kin(self.kin0, d0)
+        axpy_z(-1.0, R, q0)
+        axpy_z(-step, q0, d0)
+        d_out.zero_()
+        axpy_z(-1.0, d0, d_out)

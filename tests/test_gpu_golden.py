@@ -377,3 +377,61 @@ def test_complex_transformers_match_oracle_48(cuda, nn):
     ref = O.transform_z(2 * nn, interp, pad, skip, gd.n_c, gdc.pbc_c, phase,
                         ref)
     assert np.array_equal(i.cpu().numpy(), ref)
+
+
+def test_complex_preconditioner_at_a_k_point(cuda):
+    """The multigrid preconditioner on complex bands with kpt.phase_cd
+    follows the reference's call sequence (gpaw/preconditioner.py:34-60)
+    bit for bit against the oracle's phased stencils and transformers
+    composed with numpy, as the reference composes them (48^3, 4 bands)."""
+    import torch
+    import oracle as O
+    from gpawdft_amd.fd_operators import Laplace, guc_laplace_stencil
+    from gpawdft_amd.grid_descriptor import GridDescriptor
+    from gpawdft_amd.hamiltonian import KPoint
+    from gpawdft_amd.preconditioner import Preconditioner
+    from gpawdft_amd.transformers import transformer_paddings
+    N = np.array([48, 48, 48])
+    gd0 = GridDescriptor(N, N * 0.18 / FX['bohr'], True)
+    gd1 = gd0.coarsen()
+    gd2 = gd1.coarsen()
+    phase = np.exp(2j * np.pi * np.array([[1, -1]] * 3) *
+                   np.array([0.125, -0.25, 0.375])[:, None])
+    rng = np.random.default_rng(79)
+    R = (rng.standard_normal((4, 48, 48, 48)) +
+         1j * rng.standard_normal((4, 48, 48, 48)))
+    kin0 = Laplace(gd0, -0.5, 3, complex)
+    pc = Preconditioner(gd0, kin0, complex, block=4)
+    got = pc(torch.from_numpy(R).cuda(), KPoint(phase_cd=phase)).cpu().numpy()
+
+    def fd(gd, nn, x):
+        c, o, _ = guc_laplace_stencil(gd, -0.5, nn)
+        return O.fdz(c, o, gd.pbc_c, phase, x)
+
+    def tr(gdi, gdo, x):
+        pad, npad, skip, interp = transformer_paddings(gdi, gdo, 1)
+        return O.transform_z(2, interp, pad, skip, gdo.n_c, gdi.pbc_c,
+                             phase, x)
+
+    step = 0.66666666 / guc_laplace_stencil(gd0, -0.5, 3)[0][0]
+    assert step == pc.step
+    r1 = tr(gd0, gd1, -R)
+    d1 = 4 * step * r1
+    q1 = fd(gd1, 1, d1)
+    q1 -= r1
+    r2 = tr(gd1, gd2, q1)
+    d2 = 16 * step * r2
+    q2 = fd(gd2, 1, d2)
+    q2 -= r2
+    d2 -= 16 * step * q2
+    q1 = tr(gd2, gd1, d2)
+    d1 -= q1
+    q1 = fd(gd1, 1, d1)
+    q1 -= r1
+    d1 -= 4 * step * q1
+    d0 = tr(gd1, gd0, -d1)
+    q0 = fd(gd0, 3, d0)
+    q0 -= R
+    d0 = d0 + (-step) * q0      # axpy, two roundings
+    d0 *= -1.0
+    assert np.array_equal(got, d0)

@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kThreads) void fd_axis_z_kernel(
     const double2* __restrict__ in, double2* __restrict__ out, int n0, int n1, int n2,
     int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<double, R> cf, BlochPhases ph)
 {
-    constexpr int D = 3;                      // prefetch distance (planes)
+    constexpr int D = 1;                      // prefetch distance (planes)
     constexpr int W = 2 * R + 1 + D + ((2 * R + 1 + D) & 1);   // window slots (even)
     constexpr int LW = kTX + 2 * R;
     constexpr int LH = kTY + 2 * R;

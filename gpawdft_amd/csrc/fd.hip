@@ -444,7 +444,8 @@ static AxisCoefs<T, R> make_axis_coefs(const gpd_operator* op)
 template <int R>
 __global__ __launch_bounds__(kThreads) void fd_axis_z_kernel(
     const double2* __restrict__ in, double2* __restrict__ out, int n0, int n1, int n2,
-    int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<double, R> cf, BlochPhases ph)
+    int pbc0, int pbc1, int pbc2, int nband, int tz, AxisCoefs<double, R> cf, BlochPhases ph,
+    const double* __restrict__ v)
 {
     constexpr int D = 1;                      // prefetch distance (planes)
     constexpr int W = 2 * R + 1 + D + ((2 * R + 1 + D) & 1);   // window slots (even)
@@ -600,7 +601,14 @@ __global__ __launch_bounds__(kThreads) void fd_axis_z_kernel(
                 for (int j = 1; j <= R; j++) term(win[k][(ph_ + R + j) % W], cf.c[0][j - 1]);
 #pragma unroll
                 for (int j = 1; j <= R; j++) term(win[k][(ph_ + R - j) % W], cf.c[0][R + j - 1]);
-                b[(long)z * plane + own_off[k]] = make_double2(sr, si);
+                const long p = (long)z * plane + own_off[k];
+                if (v) {
+                    // + psi * v (the reference's Htpsit += psit * vt)
+                    const double w = v[p];
+                    sr = madd<double>(sr, x.x, w);
+                    si = madd<double>(si, x.y, w);
+                }
+                b[p] = make_double2(sr, si);
             }
         }
     }
@@ -654,7 +662,7 @@ static int launch_axis_plain(const gpd_operator* op, long nb, const T* in, T* ou
 
 template <int R>
 static int launch_axis_z_R(const gpd_operator* op, long nb, const double2* in, double2* out,
-                           const BlochPhases& ph, hipStream_t st)
+                           const BlochPhases& ph, const double* v, hipStream_t st)
 {
     const int n0 = op->n[0], n1 = op->n[1], n2 = op->n[2];
     const int tiles1 = (n1 + kTY - 1) / kTY, tiles2 = (n2 + kTX - 1) / kTX;
@@ -664,17 +672,17 @@ static int launch_axis_z_R(const gpd_operator* op, long nb, const double2* in, d
     AxisCoefs<double, R> cf = make_axis_coefs<double, R>(op);
     hipLaunchKernelGGL((fd_axis_z_kernel<R>), dim3(xcd_grid_size(nwork)), dim3(kThreads), 0, st,
                        in, out, n0, n1, n2, op->pbc[0], op->pbc[1], op->pbc[2], (int)nb, tz, cf,
-                       ph);
+                       ph, v);
     return last_error_to_status(hipGetLastError());
 }
 
 static int launch_axis_z(const gpd_operator* op, long nb, const double2* in, double2* out,
-                         const BlochPhases& ph, hipStream_t st)
+                         const BlochPhases& ph, const double* v, hipStream_t st)
 {
     switch (op->range) {
-    case 1: return launch_axis_z_R<1>(op, nb, in, out, ph, st);
-    case 2: return launch_axis_z_R<2>(op, nb, in, out, ph, st);
-    default: return launch_axis_z_R<3>(op, nb, in, out, ph, st);
+    case 1: return launch_axis_z_R<1>(op, nb, in, out, ph, v, st);
+    case 2: return launch_axis_z_R<2>(op, nb, in, out, ph, v, st);
+    default: return launch_axis_z_R<3>(op, nb, in, out, ph, v, st);
     }
 }
 
@@ -840,6 +848,39 @@ static void detect_canonical(gpd_operator* op)
     op->canonical = true;
 }
 
+// Complex apply with Bloch phases (12 host doubles or NULL for Gamma).
+// With v, out = A in + in * v in the streaming kernel; stencils it does
+// not take return kNotFused with nothing launched when v is given.
+int fd_apply_complex(const gpd_operator* op, long nin, const double2* in, double2* out,
+                     const double* phases, const double* v, hipStream_t st)
+{
+    // a neighbour wraps at most once
+    for (int c = 0; c < op->ncoefs; c++)
+        for (int ax = 0; ax < 3; ax++)
+            GPD_REQUIRE(std::abs(op->off[3 * c + ax]) <= op->n[ax], GPD_ERR_ARG,
+                        "stencil reaches beyond one periodic image");
+    BlochPhases ph;
+    for (int ax = 0; ax < 3; ax++)
+        for (int d = 0; d < 2; d++) {
+            const double* q = phases + 4 * ax + 2 * d;
+            ph.re[ax][d] = phases ? q[0] : 1.0;
+            ph.im[ax][d] = phases ? q[1] : 0.0;
+        }
+    if (axis_kernel_ok<double>(op) && op->range >= 1 && op->range <= 3)
+        return launch_axis_z(op, nin, in, out, ph, v, st);
+    if (v) return kNotFused;
+    const long nG = (long)op->n[0] * op->n[1] * op->n[2];
+    const long blocks = std::min<long>((nG + 255) / 256, 4096);
+    for (long b0 = 0; b0 < nin; b0 += 65535) {
+        const long nbb = std::min<long>(65535, nin - b0);
+        hipLaunchKernelGGL(fd_complex_kernel, dim3((unsigned)blocks, (unsigned)nbb), dim3(256), 0,
+                           st, in + b0 * nG, out + b0 * nG, op->n[0], op->n[1], op->n[2],
+                           op->pbc[0], op->pbc[1], op->pbc[2], op->ncoefs, op->d_off,
+                           op->d_coefs64, ph);
+    }
+    return last_error_to_status(hipGetLastError());
+}
+
 }  // namespace gpd
 
 using namespace gpd;
@@ -913,33 +954,9 @@ extern "C" int gpd_operator_apply(gpd_operator* op, int dtype, long nin, const v
     if (nin == 0) return GPD_OK;
     GPD_REQUIRE(d_in && d_out && d_in != d_out, GPD_ERR_ARG, "in and out must be distinct device arrays");
     hipStream_t st = (hipStream_t)stream;
-    if (dtype == GPD_C128) {
-        // a neighbour wraps at most once
-        for (int c = 0; c < op->ncoefs; c++)
-            for (int ax = 0; ax < 3; ax++)
-                GPD_REQUIRE(std::abs(op->off[3 * c + ax]) <= op->n[ax], GPD_ERR_ARG,
-                            "stencil reaches beyond one periodic image");
-        BlochPhases ph;
-        for (int ax = 0; ax < 3; ax++)
-            for (int d = 0; d < 2; d++) {
-                const double* q = (const double*)phases + 4 * ax + 2 * d;
-                ph.re[ax][d] = phases ? q[0] : 1.0;
-                ph.im[ax][d] = phases ? q[1] : 0.0;
-            }
-        if (axis_kernel_ok<double>(op) && op->range >= 1 && op->range <= 3)
-            return launch_axis_z(op, nin, (const double2*)d_in, (double2*)d_out, ph, st);
-        const long nG = (long)op->n[0] * op->n[1] * op->n[2];
-        const long blocks = std::min<long>((nG + 255) / 256, 4096);
-        for (long b0 = 0; b0 < nin; b0 += 65535) {
-            const long nbb = std::min<long>(65535, nin - b0);
-            hipLaunchKernelGGL(fd_complex_kernel, dim3((unsigned)blocks, (unsigned)nbb),
-                               dim3(256), 0, st, (const double2*)d_in + b0 * nG,
-                               (double2*)d_out + b0 * nG, op->n[0], op->n[1], op->n[2],
-                               op->pbc[0], op->pbc[1], op->pbc[2], op->ncoefs, op->d_off,
-                               op->d_coefs64, ph);
-        }
-        return last_error_to_status(hipGetLastError());
-    }
+    if (dtype == GPD_C128)
+        return fd_apply_complex(op, nin, (const double2*)d_in, (double2*)d_out,
+                                (const double*)phases, nullptr, st);
     if (dtype == GPD_F64) {
         FdEpilogue<double> ep = {};
         return fd_apply_typed<double>(op, EPI_PLAIN, nin, (const double*)d_in, (double*)d_out, ep, st);

@@ -55,4 +55,11 @@ template <typename T>
 int fd_apply_typed(const gpd_operator* op, int epi, long nb, const T* in, T* out,
                    const FdEpilogue<T>& ep, hipStream_t st);
 
+// Complex128 apply with Bloch phases (host [3][2] complex, NULL: Gamma);
+// with v (real FP64 grid) out = A in + in * v, fused when the streaming
+// kernel takes the stencil, else kNotFused is returned with nothing done.
+constexpr int kNotFused = 1;
+int fd_apply_complex(const gpd_operator* op, long nin, const double2* in, double2* out,
+                     const double* phases, const double* v, hipStream_t st);
+
 }  // namespace gpd

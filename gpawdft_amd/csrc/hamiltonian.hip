@@ -132,6 +132,30 @@ extern "C" int gpd_hamiltonian_apply(gpd_operator* kin, gpd_lfc* lfc, int dtype,
     GPD_REQUIRE(false, GPD_ERR_ARG, "unknown dtype");
 }
 
+extern "C" int gpd_hamiltonian_apply_z(gpd_operator* kin, long nb, const void* d_psi,
+                                       const void* d_v, void* d_out, const double* phases,
+                                       void* stream)
+{
+    GPD_REQUIRE(kin, GPD_ERR_ARG, "null kinetic operator");
+    GPD_REQUIRE(nb >= 0, GPD_ERR_ARG, "negative batch");
+    if (nb == 0) return GPD_OK;
+    GPD_REQUIRE(d_psi && d_v && d_out && d_psi != d_out, GPD_ERR_ARG, "bad arrays");
+    hipStream_t st = (hipStream_t)stream;
+    const double2* psi = (const double2*)d_psi;
+    const double* v = (const double*)d_v;
+    double2* out = (double2*)d_out;
+    int rc = fd_apply_complex(kin, nb, psi, out, phases, v, st);
+    if (rc != kNotFused) return rc;
+    // stencils the streaming kernel does not take: stencil, then + psi v
+    rc = fd_apply_complex(kin, nb, psi, out, phases, nullptr, st);
+    if (rc) return rc;
+    const long nG = (long)kin->n[0] * kin->n[1] * kin->n[2];
+    const long n = nb * nG;
+    hipLaunchKernelGGL(local_potential_z_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nb, nG,
+                       psi, v, out);
+    return last_error_to_status(hipGetLastError());
+}
+
 extern "C" int gpd_projector_coefs(int dtype, long nb, int Mmax, int natoms, const int* d_M_a,
                                    const int* d_ni_a, const int* d_dH_off, const void* d_dH,
                                    const void* d_P, void* d_c, void* stream)

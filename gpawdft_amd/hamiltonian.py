@@ -134,8 +134,12 @@ class Hamiltonian:
         sequence of operator calls (gpaw/hamiltonian.py:426-460)."""
         va = DeviceView(a_xG)
         vb = DeviceView(b_xG, copy_in=False)
-        wfs.kin.apply(va.t, vb.t, kpt.phase_cd)
-        self.apply_local_potential(va.t, vb.t, kpt.s)
+        if va.code == _lib.GPD_C128:
+            self.apply_kinetic_local_z(wfs.kin, va.t, vb.t, kpt.s,
+                                       kpt.phase_cd)
+        else:
+            wfs.kin.apply(va.t, vb.t, kpt.phase_cd)
+            self.apply_local_potential(va.t, vb.t, kpt.s)
         pt = wfs.pt
         nx = int(np.prod(a_xG.shape[:-3], dtype=np.int64))
         P_xM = torch.zeros((nx, pt.Mmax), dtype=va.t.dtype,
@@ -148,6 +152,21 @@ class Hamiltonian:
         c_xM = self.projector_coefs(pt, P_xM, kpt.s)
         pt.add_xM(c_xM, vb.t, q)
         vb.finish()
+
+    def apply_kinetic_local_z(self, kin, psit_xG, out_xG, s=0, phase_cd=None):
+        """out = T psit + v psit for complex bands at a k-point in one
+        launch where the stencil allows (gpd_hamiltonian_apply_z);
+        bit-identical to kin.apply then apply_local_potential."""
+        nb = int(np.prod(psit_xG.shape[:-3], dtype=np.int64))
+        vt = self._vt(s, torch.float64, psit_xG.device)
+        ph = None
+        if phase_cd is not None:
+            ph = np.ascontiguousarray(np.asarray(phase_cd, np.complex128)
+                                      .reshape(3, 2)).view(np.float64)
+        check(lib().gpd_hamiltonian_apply_z(
+            kin.handle, nb, psit_xG.data_ptr(), vt.data_ptr(),
+            out_xG.data_ptr(), None if ph is None else ph.ctypes.data,
+            stream_ptr()), 'hamiltonian_apply_z')
 
     def projector_coefs(self, pt, P_xM, s=0, out=None):
         """c_xM = P_xM dH (block diagonal over atoms), on the device."""

@@ -182,6 +182,17 @@ int gpd_hamiltonian_apply(gpd_operator* kin, gpd_lfc* lfc, int dtype, long nb,
                           const void* d_psi, const void* d_v,
                           const void* d_c, void* d_out, void* stream);
 
+/* Complex bands at a k-point: out = T psi + v psi for nb complex128 grids
+ * (interleaved re, im), the first two steps of Hamiltonian.apply
+ * (gpaw/hamiltonian.py:426-460): kin.apply(psi, out, phase_cd), then
+ * out += psi * vt.  d_v is the real FP64 potential; phases as in
+ * gpd_operator_apply.  Bit-identical to the two separate calls (the
+ * product rounded, then added); fused into the stencil's epilogue for
+ * orthorhombic stencils of range 1-3.                                  */
+int gpd_hamiltonian_apply_z(gpd_operator* kin, long nb, const void* d_psi,
+                            const void* d_v, void* d_out,
+                            const double* phases, void* stream);
+
 /* Htpsit_nG += psit_nG * vt_G for nb grids of nG points
  * (Hamiltonian.apply_local_potential, gpaw/hamiltonian.py:397-424): the
  * product is rounded, then added, as the reference's numpy expression.

@@ -276,11 +276,15 @@ def test_complex_hamiltonian_at_a_k_point(cuda):
                       pt.phase_qW[0])
     got = out.cpu().numpy()
     assert np.abs(got - ref).max() < 1e-13 * np.abs(ref).max()
-    # without the projector term the parts are bit-exact
+    # without the projector term the parts are bit-exact, as separate
+    # calls and fused in one launch
     kin_pot = torch.empty_like(d)
     wfs.kin.apply(d, kin_pot, phase)
     ham.apply_local_potential(d, kin_pot)
     assert np.array_equal(kin_pot.cpu().numpy(), t)
+    fused = torch.empty_like(d)
+    ham.apply_kinetic_local_z(wfs.kin, d, fused, 0, phase)
+    assert np.array_equal(fused.cpu().numpy(), t)
     # Hermitian: <x|H y> = <H x|y> with the Bloch boundary conditions
     x = d[:1].clone()
     y = d[1:2].clone()

@@ -400,8 +400,10 @@ __global__ __launch_bounds__(64) void lfc_integrate_z_kernel(
     *cp = acc;
 }
 
-// add: one thread per (covered point, band); per entry in the sweep's
-// volume order t = sum_m A c (from 0), then a += t * conj(phase).
+// add: one thread per (covered point, NBT bands); per entry in the sweep's
+// volume order t = sum_m A c (from 0), then a += t * conj(phase).  Each A
+// value is loaded once for the NBT bands.
+template <int NBT>
 __global__ void lfc_add_z_kernel(const double2* __restrict__ c, double2* __restrict__ a,
                                  long nG, int Mmax, long nx, long ncov,
                                  const double* __restrict__ A, const int* __restrict__ cov_G,
@@ -409,32 +411,48 @@ __global__ void lfc_add_z_kernel(const double2* __restrict__ c, double2* __restr
                                  const int* __restrict__ ent_W,
                                  const double2* __restrict__ phase_W)
 {
+    const long nbblk = (nx + NBT - 1) / NBT;
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nx * ncov) return;
+    if (idx >= nbblk * ncov) return;
     const long i = idx % ncov;
-    const long b = idx / ncov;
+    const long b0 = (idx / ncov) * NBT;
     const long G = cov_G[i];
-    double2 v = a[b * nG + G];
-    const double2* cb = c + b * Mmax;
+    double2 v[NBT];
+#pragma unroll
+    for (int bb = 0; bb < NBT; bb++)
+        v[bb] = b0 + bb < nx ? a[(b0 + bb) * nG + G] : make_double2(0.0, 0.0);
     const int e1 = cov_ptr[i + 1];
     for (int e = cov_ptr[i]; e < e1; e++) {
         const int2 en = ent[e];
         const int nm = en.y & 15;
         const double* Ar = A + en.x;
-        const double2* cm = cb + (en.y >> 4);
-        double tr = 0.0, ti = 0.0;
+        const int M = en.y >> 4;
+        double tr[NBT], ti[NBT];
+#pragma unroll
+        for (int bb = 0; bb < NBT; bb++) tr[bb] = ti[bb] = 0.0;
         for (int m = 0; m < nm; m++) {
-            tr = madd<double>(tr, Ar[m], cm[m].x);
-            ti = madd<double>(ti, Ar[m], cm[m].y);
+            const double Am = Ar[m];
+#pragma unroll
+            for (int bb = 0; bb < NBT; bb++) {
+                const long b = b0 + bb < nx ? b0 + bb : b0;
+                const double2 cm = c[b * Mmax + M + m];
+                tr[bb] = madd<double>(tr[bb], Am, cm.x);
+                ti[bb] = madd<double>(ti[bb], Am, cm.y);
+            }
         }
         const double2 ph = phase_W[ent_W[e]];
         const double pr = ph.x, pim = -ph.y;
-        const double r1 = tr * pr, r2 = ti * pim;
-        const double i1 = tr * pim, i2 = ti * pr;
-        v.x = v.x + (r1 - r2);
-        v.y = v.y + (i1 + i2);
+#pragma unroll
+        for (int bb = 0; bb < NBT; bb++) {
+            const double r1 = tr[bb] * pr, r2 = ti[bb] * pim;
+            const double i1 = tr[bb] * pim, i2 = ti[bb] * pr;
+            v[bb].x = v[bb].x + (r1 - r2);
+            v[bb].y = v[bb].y + (i1 + i2);
+        }
     }
-    a[b * nG + G] = v;
+#pragma unroll
+    for (int bb = 0; bb < NBT; bb++)
+        if (b0 + bb < nx) a[(b0 + bb) * nG + G] = v[bb];
 }
 
 // ------------------------------------------------------------------ //
@@ -961,8 +979,9 @@ extern "C" int gpd_lfc_add(gpd_lfc* l, int dtype, long nx, const void* d_c_xM, v
     if (dtype == GPD_F32)
         return lfc_add_grouped_f32(l, nx, (const float*)d_c_xM, (float*)d_a_xG, st);
     if (dtype == GPD_C128) {
-        const long n = nx * l->ncov;
-        hipLaunchKernelGGL(lfc_add_z_kernel, dim3((n + 255) / 256), dim3(256), 0, st,
+        constexpr int NBT = 4;
+        const long n = ((nx + NBT - 1) / NBT) * l->ncov;
+        hipLaunchKernelGGL((lfc_add_z_kernel<NBT>), dim3((n + 255) / 256), dim3(256), 0, st,
                            (const double2*)d_c_xM, (double2*)d_a_xG, l->nG, l->Mmax, nx, l->ncov,
                            l->A64, l->cov_G, l->cov_ptr, l->ent, l->ent_W, l->phase_W);
         return last_error_to_status(hipGetLastError());
